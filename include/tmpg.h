@@ -1,0 +1,157 @@
+/*
+ * tmpg.h -- C ABI of the MI355X per-step PIC library (libtmpg.so).
+ *
+ * Replaces the reference's argument-less Fortran module procedures on the
+ * per-step hot path (called from code/tristanmainloop.F90:117-242) by plain
+ * C entry points over an opaque context.  The Fortran host binds them with
+ * bind(C) (tristan-mp-pu-master_densdecomp_amd/fortran/m_tmpg.F90).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a negative TMPG_E* code; it
+ *    never aborts.  tmpg_last_error() gives the message.
+ *  - Host arrays are borrowed only for the duration of the call.
+ *  - Grid arrays at the boundary use the reference layout: Fortran
+ *    column-major (mx,my,mz) including ghost cells, mx = nx + nghost,
+ *    nghost = 5 (order 1) or 7 (orders 2, 3); in 2D mz = 1
+ *    (code/fields.F90:166-184, 221-229).  Inside the library the grid is the
+ *    periodic interior only; ghost layers returned by the getters hold the
+ *    periodic images the reference's bc_b1/bc_e1 would put there.
+ *  - Particle coordinates at the boundary are the reference's absolute
+ *    1-based coordinates (interior x in [nghost/2+1, nghost/2+1+nx)).
+ *    Inside, a particle is an integer cell plus an FP32 offset in [0,1).
+ *  - One HIP stream per context; calls are stream-ordered.  Getters
+ *    synchronise.  One host thread per context.
+ */
+#ifndef TMPG_H
+#define TMPG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TMPG_OK            0
+#define TMPG_EINVAL       -1   /* bad argument */
+#define TMPG_ENOMEM       -2   /* device allocation failed */
+#define TMPG_EHIP         -3   /* HIP runtime error */
+#define TMPG_EOVERFLOW    -4   /* particle capacity exceeded */
+#define TMPG_ESTATE       -5   /* call not valid in the current state */
+
+/* reference quirks reproduced by default (compat bits) */
+#define TMPG_Q_GATHER2   1  /* 2nd-order gather: primal weights over the dual range
+                               (code/particles_movedeposit.F90:709-832) */
+#define TMPG_Q_BXY_Z     2  /* 3D primal Bx/By not averaged in z
+                               (code/particles_movedeposit.F90:988-989) */
+#define TMPG_COMPAT_DEFAULT (TMPG_Q_GATHER2 | TMPG_Q_BXY_Z)
+
+typedef struct tmpg_config {
+    int32_t dim;          /* 2 or 3 (reference: -DtwoD at compile time) */
+    int32_t order;        /* shape order 1, 2, 3 (reference: -Ddd1/-Ddd2/-Ddd3) */
+    int32_t nx, ny, nz;   /* interior cells (<grid> mx0, my0, mz0); nz = 1 in 2D */
+    int32_t ntimes;       /* filter passes (<algorithm> ntimes) */
+    int32_t compat;       /* TMPG_Q_* bits */
+    int32_t track_ids;    /* nonzero: carry a per-particle int32 id (like p%ind) */
+    float   c;            /* Courant number (<time> c) */
+    float   corr;         /* speed-of-light correction (<algorithm> Corr) */
+    float   qm[2];        /* q/m of species 0 (ions) and 1 (electrons): qmi, qme */
+    float   q[2];         /* charge per macro-particle: qi, qe */
+    float   mass[2];      /* mi, me after normalisation (energy diagnostics) */
+    int64_t capacity[2];  /* particle capacity per species (0: sized at first load) */
+    int32_t tile[3];      /* cell tile for the sorted push (0: default) */
+    int32_t reserved[5];
+} tmpg_config;
+
+typedef struct tmpg_ctx tmpg_ctx;
+
+/* life cycle --------------------------------------------------------- */
+/* replaces initialize()'s allocate_fields/allocate_particles
+ * (code/fields.F90:221-377, code/particles.F90:286-351) */
+int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out);
+int tmpg_destroy(tmpg_ctx* ctx);
+/* message of the last failing call (ctx may be NULL for tmpg_create) */
+const char* tmpg_last_error(const tmpg_ctx* ctx);
+int tmpg_sync(tmpg_ctx* ctx);
+
+/* state transfer ----------------------------------------------------- */
+/* ex..bz module arrays (code/fields.F90:81-86, 346-351) */
+int tmpg_set_fields(tmpg_ctx* ctx, const float* ex, const float* ey, const float* ez,
+                    const float* bx, const float* by, const float* bz);
+int tmpg_get_fields(tmpg_ctx* ctx, float* ex, float* ey, float* ez,
+                    float* bx, float* by, float* bz);
+/* curx..curz (code/fields.F90:346) */
+int tmpg_set_currents(tmpg_ctx* ctx, const float* jx, const float* jy, const float* jz);
+int tmpg_get_currents(tmpg_ctx* ctx, float* jx, float* jy, float* jz);
+/* p(:) of species 0 (ions, p(1:ions)) or 1 (electrons, p(maxhlf+1:)):
+ * x,y,z,u,v,w,ch (code/particles.F90:51-55).  ch may be NULL (all 1);
+ * id may be NULL.  set replaces the species' particles. */
+int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n,
+                       const float* x, const float* y, const float* z,
+                       const float* u, const float* v, const float* w,
+                       const float* ch, const int32_t* id);
+int tmpg_count_particles(tmpg_ctx* ctx, int species, int64_t* n);
+/* writes min(n_capacity, count) particles in the library's order */
+int tmpg_get_particles(tmpg_ctx* ctx, int species, int64_t n_capacity, int64_t* n,
+                       float* x, float* y, float* z, float* u, float* v, float* w,
+                       float* ch, int32_t* id);
+
+/* per-step kernels (reference names in brackets) -------------------- */
+int tmpg_advance_b_half(tmpg_ctx* ctx);   /* [advance_Bhalfstep] code/fields.F90:586-728 */
+int tmpg_advance_e(tmpg_ctx* ctx);        /* [advance_Efield]    code/fields.F90:739-870 */
+int tmpg_reset_currents(tmpg_ctx* ctx);   /* [reset_currents]    code/fields.F90:499-506 */
+/* [move_particles] code/particles_movedeposit.F90:63-88: gather + Boris +
+ * position advance of both species, no deposit */
+int tmpg_move(tmpg_ctx* ctx);
+/* [deposit_particles] code/particles_movedeposit.F90:1281-2051: Esirkepov
+ * current of the last tmpg_move (old positions are the ones before it),
+ * added to J, with periodic wrap of positions */
+int tmpg_deposit(tmpg_ctx* ctx);
+/* fused move + deposit (one pass over the particles): the hot path */
+int tmpg_push_deposit(tmpg_ctx* ctx);
+/* [exchange_current] code/fieldboundaries.F90:1768-2189.  The library
+ * deposits with periodic wrap, so the fold is already done: a no-op kept
+ * for call-sequence parity */
+int tmpg_fold_currents(tmpg_ctx* ctx);
+/* [apply_filter1_opt] code/filter.F90:8-221: npasses binomial passes */
+int tmpg_filter(tmpg_ctx* ctx, int npasses);
+int tmpg_add_current(tmpg_ctx* ctx);      /* [add_current] code/fields.F90:1372-1395 */
+/* nsteps of the main loop (code/tristanmainloop.F90:107-274, periodic box):
+ * B half, push+deposit, B half, E, filter (or denoiser), add current */
+int tmpg_step(tmpg_ctx* ctx, int nsteps);
+
+/* diagnostics (FP64 on the device) ------------------------------------ */
+/* out[0] = 0.5*sum(E^2), out[1] = 0.5*sum(B^2) over the interior,
+ * out[2], out[3] = kinetic energy sum(ch*m*c^2*(gamma-1)) of species 0, 1,
+ * out[4] = particles of species 0, out[5] = of species 1,
+ * out[6] = max |rho - rho_prev - div cur| of the last deposit (continuity,
+ *          needs tmpg_track_continuity(ctx,1) before the step), else -1,
+ * out[7] = L2 norm of rho used for that check */
+int tmpg_diag(tmpg_ctx* ctx, double out[8]);
+/* charge density at the nodes (interior nx*ny*nz, x fastest), FP64,
+ * deposited with the order's own shape from the current positions */
+int tmpg_charge_density(tmpg_ctx* ctx, double* rho);
+int tmpg_track_continuity(tmpg_ctx* ctx, int on);
+
+/* plasma loader (next to code/particles.F90:2549-2938) ----------------- */
+/* appends ppc particles per cell of one species, uniform positions,
+ * drift gamma*beta along x (sign), isotropic thermal spread with
+ * temperature theta = kT/mc^2 (Maxwell-Juttner, boosted), counter-based RNG
+ * keyed by seed. */
+int tmpg_load_uniform(tmpg_ctx* ctx, int species, int ppc, float drift_gb,
+                      float theta, uint64_t seed);
+
+/* timing: milliseconds of the last tmpg_step per phase:
+ * [0] fields, [1] push+deposit, [2] filter/denoise, [3] sort, [4] total */
+int tmpg_last_timings(tmpg_ctx* ctx, double ms[5]);
+
+/* input file: the reference's <block> / key = value # comment format
+ * (code/par.c) */
+typedef struct tmpg_input tmpg_input;
+int tmpg_input_open(const char* path, tmpg_input** out);
+int tmpg_input_close(tmpg_input* in);
+int tmpg_input_get_int(const tmpg_input* in, const char* block, const char* key, int def, int* val);
+int tmpg_input_get_double(const tmpg_input* in, const char* block, const char* key, double def, double* val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,253 @@
+"""HIP path (libtmpg.so through its C ABI) against the reference and the oracle.
+
+Bars (norm-wise relative error, FP32 GPU vs an FP64 yardstick):
+  * one main-loop step on the golden fixtures vs the compiled reference's
+    own FP64 build (tests/golden): E, B, J <= 2e-6, momenta <= 1e-6,
+    positions within 2e-6 cells;
+  * push+deposit / filter / Yee on larger seeded states vs the FP64 oracle,
+    same bars;
+  * the split move + deposit equals the fused push bit for bit.
+The GPU keeps positions as cell + FP32 offset and the reference as an
+absolute FP32 coordinate, so the FP32 reference itself is further from FP64
+than the GPU is; FP64 is the common yardstick.  Deposit atomics reorder the
+FP32 sums, hence norm-wise bars rather than bitwise ones.
+"""
+import numpy as np
+import pytest
+
+import orc
+from gpu_util import (load_fixture, make_context, norm64, periodic_diff, random_state, rel, sorted_by_id)
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(d, o) for d in (2, 3) for o in (1, 2, 3)]
+TOL_FIELD = 2e-6
+TOL_MOM = 1e-6
+TOL_POS = 2e-6
+
+
+@pytest.fixture(scope="module")
+def tmpg():
+    import tmpg as t
+    t.load_library()
+    return t
+
+
+def _interior(ctx, a):
+    h, hz = ctx.ng // 2, ctx.ngz // 2
+    if ctx.dim == 3:
+        return a[..., hz:hz + ctx.nz, h:h + ctx.ny, h:h + ctx.nx]
+    return a[..., :, h:h + ctx.ny, h:h + ctx.nx]
+
+
+def _check_particles(ctx, got, ids, ref, what):
+    g = sorted_by_id(got, ids)
+    assert rel(g[3:6], ref[3:6]) < TOL_MOM, f"{what} momenta {rel(g[3:6], ref[3:6])}"
+    h, hz = ctx.ng // 2, ctx.ngz // 2
+    dx = periodic_diff(g[0], ref[0], h + 1, ctx.nx)
+    dy = periodic_diff(g[1], ref[1], h + 1, ctx.ny)
+    err = max(np.abs(dx).max(), np.abs(dy).max())
+    if ctx.dim == 3:
+        err = max(err, np.abs(periodic_diff(g[2], ref[2], hz + 1, ctx.nz)).max())
+    # absolute-coordinate outputs are FP32 near 10..50: 1 ulp ~ 4e-6
+    assert err < 8e-6, f"{what} positions {err}"
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+def test_step_vs_reference_fp64(tmpg, dim, order):
+    """one main-loop step against the compiled reference's FP64 build"""
+    fx, p = load_fixture(dim, order)
+    ctx = make_context(tmpg, dim, order, p["nx"], p["ny"], p["nz"], p, ntimes=p["ntimes"])
+    cfg = orc.make_cfg(dim, order, p["nx"], p["ny"], p["nz"])
+    ctx.set_fields(orc.fill_from_interior(cfg, fx["f_in"], np.float32)[:6])
+    ctx.set_particles(0, fx["pi_in"])
+    ctx.set_particles(1, fx["pe_in"])
+    ctx.step(1)
+    f = _interior(ctx, ctx.get_fields())
+    j = _interior(ctx, ctx.get_currents())
+    ref = fx["step_f64"]
+    e_e, e_b, e_j = rel(f[:3], ref[:3]), rel(f[3:6], ref[3:6]), rel(j, ref[6:9])
+    print(f"dim{dim} ord{order}: E {e_e:.2e} B {e_b:.2e} J {e_j:.2e}")
+    assert e_e < TOL_FIELD and e_b < TOL_FIELD and e_j < TOL_FIELD
+    for s, key in ((0, "step_pi_f64"), (1, "step_pe_f64")):
+        got, ids = ctx.get_particles(s, with_ids=True)
+        _check_particles(ctx, got, ids, fx[key], f"species {s}")
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+def test_push_deposit_vs_oracle_fp64(tmpg, dim, order):
+    """fused push+deposit (no filter) on a larger random state"""
+    if dim == 3:
+        nx, ny, nz, npart = 16, 12, 10, 6000
+    else:
+        nx, ny, nz, npart = 48, 40, 1, 8000
+    p = dict(c=0.45, corr=1.025, c_omp=4.0, gamma0=0.5, ppc0=4.0, me=1.0, mi=16.0)
+    f, pi, pe = random_state(dim, order, nx, ny, nz, npart, seed=7 + 10 * dim + order)
+    cfg = orc.make_cfg(dim, order, nx, ny, nz)
+    f6 = orc.fill_from_interior(cfg, f.astype(np.float64), np.float64)
+    nm = norm64(p)
+    ctx = make_context(tmpg, dim, order, nx, ny, nz, p)
+    ctx.set_fields(orc.fill_from_interior(cfg, f, np.float32)[:6])
+    ctx.set_particles(0, pi)
+    ctx.set_particles(1, pe)
+    ctx.reset_currents()
+    ctx.push_deposit()
+    # oracle: mover then deposit + fold, FP64
+    ri, re = pi.astype(np.float64), pe.astype(np.float64)
+    orc.mover(cfg, f6, ri, nm["qmi"])
+    orc.mover(cfg, f6, re, nm["qme"])
+    j3 = orc.zeros_grid(cfg, 3, np.float64)
+    orc.deposit(cfg, j3, ri, nm["qi"])
+    orc.deposit(cfg, j3, re, nm["qe"])
+    orc.fold(cfg, j3)
+    sl = orc.interior_slices(cfg)
+    jg = _interior(ctx, ctx.get_currents())
+    e_j = rel(jg, j3[(slice(None),) + sl])
+    print(f"dim{dim} ord{order}: J {e_j:.2e}")
+    assert e_j < TOL_FIELD
+    for s, ref in ((0, ri), (1, re)):
+        got, ids = ctx.get_particles(s, with_ids=True)
+        _check_particles(ctx, got, ids, ref, f"species {s}")
+
+
+@pytest.mark.parametrize("dim,order", [(2, 2), (3, 3), (3, 1)])
+def test_split_equals_fused(tmpg, dim, order):
+    nx, ny, nz = (16, 12, 10) if dim == 3 else (32, 24, 1)
+    p = dict(c=0.45, corr=1.025, c_omp=4.0, gamma0=0.5, ppc0=4.0, me=1.0, mi=4.0)
+    f, pi, pe = random_state(dim, order, nx, ny, nz, 3000, seed=3)
+    cfg = orc.make_cfg(dim, order, nx, ny, nz)
+    f32 = orc.fill_from_interior(cfg, f, np.float32)[:6]
+    outs = []
+    for split in (False, True):
+        ctx = make_context(tmpg, dim, order, nx, ny, nz, p)
+        ctx.set_fields(f32)
+        ctx.set_particles(0, pi)
+        ctx.set_particles(1, pe)
+        ctx.reset_currents()
+        if split:
+            ctx.move()
+            ctx.deposit()
+        else:
+            ctx.push_deposit()
+        outs.append((ctx.get_particles(0), ctx.get_particles(1), ctx.get_currents()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    assert rel(outs[1][2], outs[0][2]) < 1e-6   # atomics may reorder
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_filter_vs_oracle(tmpg, dim):
+    nx, ny, nz = (20, 18, 14) if dim == 3 else (64, 40, 1)
+    order = 3
+    cfg = orc.make_cfg(dim, order, nx, ny, nz)
+    rng = np.random.default_rng(11)
+    jin = rng.standard_normal((3, nz if dim == 3 else 1, ny, nx)).astype(np.float32)
+    jp = orc.fill_from_interior(cfg, jin, np.float32)
+    p = dict(c=0.45, corr=1.025, c_omp=4.0, gamma0=0.5, ppc0=4.0, me=1.0, mi=4.0)
+    ctx = make_context(tmpg, dim, order, nx, ny, nz, p)
+    ctx.set_currents(jp)
+    ctx.filter(4)
+    got = _interior(ctx, ctx.get_currents())
+    ref = jp.astype(np.float64).copy()
+    orc.filt(cfg, ref, 4)
+    e = rel(got, ref[(slice(None),) + orc.interior_slices(cfg)])
+    assert e < 1e-6, e
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_yee_vs_oracle(tmpg, dim):
+    nx, ny, nz = (20, 18, 14) if dim == 3 else (64, 40, 1)
+    order = 2
+    cfg = orc.make_cfg(dim, order, nx, ny, nz)
+    f, _, _ = random_state(dim, order, nx, ny, nz, 1, seed=5)
+    p = dict(c=0.45, corr=1.025, c_omp=4.0, gamma0=0.5, ppc0=4.0, me=1.0, mi=4.0)
+    ctx = make_context(tmpg, dim, order, nx, ny, nz, p)
+    f32 = orc.fill_from_interior(cfg, f, np.float32)[:6]
+    ctx.set_fields(f32)
+    ref = np.ascontiguousarray(orc.fill_from_interior(cfg, f.astype(np.float64), np.float64)[:6])
+    for _ in range(3):
+        ctx.advance_b_half()
+        ctx.advance_b_half()
+        ctx.advance_e()
+        for blk in (3, 0):
+            a = np.ascontiguousarray(ref[blk:blk + 3])
+            orc.ghost_fill(cfg, a)
+            ref[blk:blk + 3] = a
+        orc.bhalf(cfg, ref)
+        b = np.ascontiguousarray(ref[3:6]); orc.ghost_fill(cfg, b); ref[3:6] = b
+        orc.bhalf(cfg, ref)
+        b = np.ascontiguousarray(ref[3:6]); orc.ghost_fill(cfg, b); ref[3:6] = b
+        orc.efull(cfg, ref)
+    got = _interior(ctx, ctx.get_fields())
+    e = rel(got, ref[(slice(None),) + orc.interior_slices(cfg)])
+    assert e < 1e-6, e
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+def test_continuity_residual(tmpg, dim, order):
+    """rho_new - rho_old = div_b(cur) to FP32 round-off (FP64 check on the device)"""
+    nx, ny, nz = (16, 16, 12) if dim == 3 else (48, 32, 1)
+    p = dict(c=0.45, corr=1.025, c_omp=4.0, gamma0=0.5, ppc0=4.0, me=1.0, mi=4.0)
+    f, pi, pe = random_state(dim, order, nx, ny, nz, 5000, seed=21)
+    cfg = orc.make_cfg(dim, order, nx, ny, nz)
+    ctx = make_context(tmpg, dim, order, nx, ny, nz, p)
+    ctx.set_fields(orc.fill_from_interior(cfg, f, np.float32)[:6])
+    ctx.set_particles(0, pi)
+    ctx.set_particles(1, pe)
+    ctx.track_continuity(True)
+    ctx.reset_currents()
+    ctx.push_deposit()
+    d = ctx.diag()
+    maxres, rnorm = d[6], d[7]
+    # per-node charge scale
+    qscale = rnorm / np.sqrt(nx * ny * nz)
+    print(f"dim{dim} ord{order}: max residual {maxres:.3e}, rho rms {qscale:.3e}")
+    assert maxres >= 0
+    assert maxres < 2e-6 * qscale
+
+
+def test_fields_roundtrip(tmpg):
+    p = dict(c=0.45, corr=1.025, c_omp=4.0, gamma0=0.5, ppc0=4.0, me=1.0, mi=4.0)
+    ctx = make_context(tmpg, 3, 3, 12, 10, 8, p)
+    cfg = orc.make_cfg(3, 3, 12, 10, 8)
+    f, _, _ = random_state(3, 3, 12, 10, 8, 1, seed=1)
+    padded = orc.fill_from_interior(cfg, f, np.float32)[:6]
+    ctx.set_fields(padded)
+    got = ctx.get_fields()
+    np.testing.assert_array_equal(_interior(ctx, got), f)
+    # ghost layers the reference fills (1..h, m-h..m-1) hold the periodic images
+    sl = (slice(None), slice(0, ctx.mz - 1), slice(0, ctx.my - 1), slice(0, ctx.mx - 1))
+    np.testing.assert_array_equal(got[sl], padded[sl])
+
+
+def test_loader_moments(tmpg):
+    """drifting Maxwell-Juttner loader: lab-frame <u_x> = Gamma*beta*K3/K2(1/theta)
+    and <gamma> = Gamma*K3/K2 - theta/Gamma (moments of the boosted Juttner
+    distribution), isotropic transverse spread, positions inside the grid"""
+    from scipy.special import kve
+    p = dict(c=0.45, corr=1.025, c_omp=4.0, gamma0=0.5, ppc0=4.0, me=1.0, mi=1.0)
+    ctx = make_context(tmpg, 3, 2, 32, 32, 32, p, track_ids=False)
+    for theta, gb in ((1e-2, 0.5), (0.3, 1.0)):
+        ctx.set_particles(0, np.zeros((7, 0), np.float32))
+        ctx.set_particles(1, np.zeros((7, 0), np.float32))
+        ctx.load_uniform(0, 8, gb, theta, 1)
+        ctx.load_uniform(1, 8, -gb, theta, 2)
+        assert ctx.count(0) == 8 * 32 ** 3
+        a = ctx.get_particles(0)
+        b = ctx.get_particles(1)
+        G = np.sqrt(1 + gb * gb)
+        hk = kve(3, 1 / theta) / kve(2, 1 / theta)
+        ux_mean = gb * hk
+        g_mean = G * hk - theta / G
+        n = a.shape[1]
+        for prt, sgn in ((a, 1), (b, -1)):
+            u = prt[3:6].astype(np.float64)
+            gam = np.sqrt(1 + (u ** 2).sum(0))
+            assert abs(u[0].mean() - sgn * ux_mean) < 5 * u[0].std() / np.sqrt(n) + 1e-4
+            assert abs(gam.mean() - g_mean) < 5 * gam.std() / np.sqrt(n) + 1e-4
+            assert abs(u[1].std() - u[2].std()) < 0.01 * u[1].std()
+        h = 4
+        for ax in range(3):
+            # FP32 absolute coordinates may round up onto the upper edge, which
+            # the reference keeps in the domain (it wraps only x > maxx)
+            assert a[ax].min() >= h and a[ax].max() <= h + 32

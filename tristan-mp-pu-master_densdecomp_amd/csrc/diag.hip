@@ -1,0 +1,325 @@
+// diag.hip -- FP64 diagnostics on the device and the plasma loader.
+//
+// Energies and the charge-continuity residual are new (the reference's
+// energy() at code/tristanmainloop.F90:384-494 is never called); the charge
+// density uses the deposit's own B-spline at integer nodes, so the
+// Esirkepov identity rho_new - rho_old = div_b(cur) holds to round-off.
+// The loader follows the reference's inject_plasma_region/maxwell_dist
+// (code/particles.F90:2549-2938): uniform positions per cell, a tabulated
+// Maxwell-Juttner speed distribution (init_maxw_table) and Zenitani's
+// flipping method for the drift.
+#include <vector>
+#include <cmath>
+
+#include "pic_math.h"
+#include "tmpg_internal.h"
+
+namespace tmpg {
+namespace {
+
+__device__ __forceinline__ int wrapd(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < BS / 64; ++k) t += sh[k];
+    return t;
+}
+
+__global__ void __launch_bounds__(256) k_sumsq(const float* __restrict__ a, const float* __restrict__ b,
+                                               const float* __restrict__ c, int64_t n, double* out)
+{
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        double x = a[t], y = b[t], z = c[t];
+        s += x * x + y * y + z * z;
+    }
+    s = block_sum<256>(s, sh);
+    if (threadIdx.x == 0) atomicAdd(out, 0.5 * s);
+}
+
+__global__ void __launch_bounds__(256) k_kinetic(const float* __restrict__ u, const float* __restrict__ v,
+                                                 const float* __restrict__ w, const float* __restrict__ ch,
+                                                 int64_t n, double mc2, double* out)
+{
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        double uu = (double)u[t] * u[t] + (double)v[t] * v[t] + (double)w[t] * w[t];
+        double gm1 = uu / (sqrt(1.0 + uu) + 1.0);
+        s += (ch ? (double)ch[t] : 1.0) * gm1;
+    }
+    s = block_sum<256>(s, sh);
+    if (threadIdx.x == 0) atomicAdd(out, mc2 * s);
+}
+
+template <int DIM, int ORD>
+__global__ void __launch_bounds__(256) k_rho(const int32_t* __restrict__ cell, const float* __restrict__ ox,
+                                             const float* __restrict__ oy, const float* __restrict__ oz,
+                                             const float* __restrict__ ch, int64_t n, PackInfo pk, int nx, int ny,
+                                             int nz, double q, double* __restrict__ rho)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int i, j, k;
+    unpack_cell(pk, cell[p], i, j, k);
+    using P = Piece<ORD>;
+    float wx[P::cnt], wy[P::cnt], wz[P::cnt];
+    const int mx = piece_of<ORD>(ox[p]), my = piece_of<ORD>(oy[p]);
+    P::w(ox[p] - (float)mx, wx);
+    P::w(oy[p] - (float)my, wy);
+    int mz = 0;
+    if (DIM == 3) {
+        mz = piece_of<ORD>(oz[p]);
+        P::w(oz[p] - (float)mz, wz);
+    }
+    const double qq = ch ? q * (double)ch[p] : q;
+    const int nzr = (DIM == 3) ? P::cnt : 1;
+    for (int c = 0; c < nzr; ++c) {
+        int kk = (DIM == 3) ? wrapd(k + mz + P::first + c, nz) : 0;
+        double wzz = (DIM == 3) ? (double)wz[c] : 1.0;
+        for (int b = 0; b < P::cnt; ++b) {
+            int jj = wrapd(j + my + P::first + b, ny);
+            for (int a = 0; a < P::cnt; ++a) {
+                int ii = wrapd(i + mx + P::first + a, nx);
+                atomicAdd(rho + ii + (int64_t)nx * (jj + (int64_t)ny * kk),
+                          qq * (double)wx[a] * (double)wy[b] * wzz);
+            }
+        }
+    }
+}
+
+// residual r = rho_new - rho_old - div_b(cur); max |r| and sum rho^2
+__global__ void __launch_bounds__(256) k_continuity(const double* __restrict__ rn, const double* __restrict__ ro,
+                                                    const float* __restrict__ jx, const float* __restrict__ jy,
+                                                    const float* __restrict__ jz, int dim, int nx, int ny, int nz,
+                                                    double* out)
+{
+    __shared__ double sh[4];
+    const int64_t n = (int64_t)nx * ny * nz;
+    double mx = 0.0, s2 = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i = (int)(t % nx);
+        int64_t r = t / nx;
+        int j = (int)(r % ny);
+        int k = (int)(r / ny);
+        int im = i == 0 ? nx - 1 : i - 1, jm = j == 0 ? ny - 1 : j - 1;
+        double div = ((double)jx[t] - jx[im + (int64_t)nx * (j + (int64_t)ny * k)]) +
+                     ((double)jy[t] - jy[i + (int64_t)nx * (jm + (int64_t)ny * k)]);
+        if (dim == 3) {
+            int km = k == 0 ? nz - 1 : k - 1;
+            div += (double)jz[t] - jz[i + (int64_t)nx * (j + (int64_t)ny * km)];
+        }
+        double res = fabs(rn[t] - ro[t] - div);
+        mx = res > mx ? res : mx;
+        s2 += rn[t] * rn[t];
+    }
+    // max over the block
+    for (int o = 32; o > 0; o >>= 1) {
+        double v = __shfl_down(mx, o, 64);
+        mx = v > mx ? v : mx;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sh[wv] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int k = 0; k < 4; ++k) m = sh[k] > m ? sh[k] : m;
+        // non-negative doubles order like their bit patterns
+        atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(m));
+    }
+    __syncthreads();
+    s2 = block_sum<256>(s2, sh);
+    if (threadIdx.x == 0) atomicAdd(out + 1, s2);
+}
+
+// ------------------------------------------------------------------ loader
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float urand(uint64_t key, uint32_t draw)
+{
+    uint64_t r = mix64(key ^ mix64(0x5851f42d4c957f2dull + draw));
+    return ((float)(r >> 40) + 0.5f) * (1.0f / 16777216.0f);   // (0,1)
+}
+
+__global__ void __launch_bounds__(256) k_load(int32_t* cell, float* ox, float* oy, float* oz, float* u, float* v,
+                                              float* w, int32_t* id, int64_t first, int64_t count, int ppc,
+                                              int dim, int nx, int ny, PackInfo pk, float gdrift,
+                                              const float* __restrict__ cdf_u, int ntab, uint64_t seed,
+                                              int32_t id0)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t c = t / ppc;
+    const int i = (int)(c % nx);
+    const int64_t r = c / nx;
+    const int j = (int)(r % ny);
+    const int k = (int)(r / ny);
+    const uint64_t key = mix64(seed * 0x2545F4914F6CDD1Dull + (uint64_t)t);
+    const int64_t p = first + t;
+    cell[p] = pack_cell(pk, i, j, k);
+    ox[p] = fminf(urand(key, 0), 0.99999994f);
+    oy[p] = fminf(urand(key, 1), 0.99999994f);
+    if (dim == 3) oz[p] = fminf(urand(key, 2), 0.99999994f);
+    // |u'| from the tabulated Maxwell-Juttner CDF (uniform grid in CDF)
+    float x = urand(key, 3) * (float)(ntab - 1);
+    int b = (int)x;
+    if (b >= ntab - 1) b = ntab - 2;
+    float fr = x - (float)b;
+    float um = cdf_u[b] + fr * (cdf_u[b + 1] - cdf_u[b]);
+    // isotropic direction
+    float ct = 2.0f * urand(key, 4) - 1.0f;
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float ph = 6.2831853f * urand(key, 5);
+    float ux = um * ct, uy = um * st * cosf(ph), uz = um * st * sinf(ph);
+    // Zenitani's flipping, then boost along x with gamma*beta = gdrift
+    if (gdrift != 0.0f) {
+        float G = sqrtf(1.0f + gdrift * gdrift);
+        float beta = gdrift / G;
+        float gp = sqrtf(1.0f + ux * ux + uy * uy + uz * uz);
+        if (-beta * ux / gp > urand(key, 6)) ux = -ux;
+        ux = G * (ux + beta * gp);
+    }
+    u[p] = ux;
+    v[p] = uy;
+    w[p] = uz;
+    if (id) id[p] = id0 + (int32_t)t;
+}
+
+}  // namespace
+
+int diag_energies(tmpg_ctx* ctx, double out[4])
+{
+    double* d = ctx->red;
+    TMPG_HIP_CHECK(ctx, hipMemsetAsync(d, 0, 4 * sizeof(double), ctx->stream));
+    const int64_t n = ctx->g.n;
+    int blocks = (int)std::min<int64_t>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_sumsq, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[EX], ctx->f[EY], ctx->f[EZ], n, d);
+    hipLaunchKernelGGL(k_sumsq, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[BX], ctx->f[BY], ctx->f[BZ], n,
+                       d + 1);
+    for (int s = 0; s < 2; ++s) {
+        Species& sp = ctx->sp[s];
+        if (sp.n == 0) continue;
+        int pb = (int)std::min<int64_t>((sp.n + 255) / 256, 4096);
+        double mc2 = (double)ctx->cfg.mass[s] * (double)ctx->cfg.c * (double)ctx->cfg.c;
+        hipLaunchKernelGGL(k_kinetic, dim3(pb), dim3(256), 0, ctx->stream, sp.u, sp.v, sp.w,
+                           sp.has_ch ? sp.ch : nullptr, sp.n, mc2, d + 2 + s);
+    }
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(out, d, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TMPG_OK;
+}
+
+int diag_rho(tmpg_ctx* ctx, double* rho)
+{
+    const Grid& g = ctx->g;
+    TMPG_HIP_CHECK(ctx, hipMemsetAsync(rho, 0, g.n * sizeof(double), ctx->stream));
+    for (int s = 0; s < 2; ++s) {
+        Species& sp = ctx->sp[s];
+        if (sp.n == 0) continue;
+        dim3 grid((unsigned)((sp.n + 255) / 256));
+        const double q = ctx->cfg.q[s];
+        const float* ch = sp.has_ch ? sp.ch : nullptr;
+#define TMPG_RHO(D, O)                                                                                       \
+    hipLaunchKernelGGL((k_rho<D, O>), grid, dim3(256), 0, ctx->stream, sp.cell, sp.ox, sp.oy, sp.oz, ch, sp.n, \
+                       ctx->pk, g.nx, g.ny, g.nz, q, rho)
+        switch (g.dim * 10 + ctx->cfg.order) {
+        case 21: TMPG_RHO(2, 1); break;
+        case 22: TMPG_RHO(2, 2); break;
+        case 23: TMPG_RHO(2, 3); break;
+        case 31: TMPG_RHO(3, 1); break;
+        case 32: TMPG_RHO(3, 2); break;
+        case 33: TMPG_RHO(3, 3); break;
+        }
+#undef TMPG_RHO
+        TMPG_HIP_CHECK(ctx, hipGetLastError());
+    }
+    return TMPG_OK;
+}
+
+int diag_continuity(tmpg_ctx* ctx, double* maxres, double* rhonorm)
+{
+    const Grid& g = ctx->g;
+    int rc = diag_rho(ctx, ctx->rho);
+    if (rc) return rc;
+    double* d = ctx->red + 4;
+    TMPG_HIP_CHECK(ctx, hipMemsetAsync(d, 0, 2 * sizeof(double), ctx->stream));
+    int blocks = (int)std::min<int64_t>((g.n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_continuity, dim3(blocks), dim3(256), 0, ctx->stream, ctx->rho, ctx->rho_prev, ctx->f[JX],
+                       ctx->f[JY], ctx->f[JZ], g.dim, g.nx, g.ny, g.nz, d);
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    double h[2];
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(h, d, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *maxres = h[0];
+    *rhonorm = std::sqrt(h[1]);
+    return TMPG_OK;
+}
+
+// Maxwell-Juttner speed table: inverse CDF of f(u) ~ u^2 exp(-(gamma-1)/theta)
+// sampled on a uniform grid of the CDF (the reference's init_maxw_table,
+// code/particles.F90:2593-2660, tabulates the same distribution)
+static std::vector<float> mj_inverse_cdf(double theta, int ntab)
+{
+    std::vector<float> out(ntab, 0.0f);
+    if (theta <= 0.0) return out;
+    const int nq = 200000;
+    // support: up to where the density has fallen by e^-40
+    double umax = std::sqrt(std::pow(1.0 + 40.0 * theta, 2) - 1.0);
+    std::vector<double> cdf(nq + 1, 0.0);
+    double du = umax / nq;
+    auto f = [&](double u) { double g = std::sqrt(1.0 + u * u); return u * u * std::exp(-(u * u / (g + 1.0)) / theta); };
+    for (int k = 1; k <= nq; ++k) cdf[k] = cdf[k - 1] + 0.5 * du * (f((k - 1) * du) + f(k * du));
+    double tot = cdf[nq];
+    int k = 0;
+    for (int t = 0; t < ntab; ++t) {
+        double target = tot * (double)t / (ntab - 1);
+        while (k < nq && cdf[k + 1] < target) ++k;
+        double seg = cdf[k + 1] - cdf[k];
+        double fr = seg > 0 ? (target - cdf[k]) / seg : 0.0;
+        if (fr > 1.0) fr = 1.0;
+        out[t] = (float)((k + fr) * du);
+    }
+    return out;
+}
+
+int particles_load_uniform(tmpg_ctx* ctx, int s, int ppc, float drift, float theta, uint64_t seed)
+{
+    Species& sp = ctx->sp[s];
+    const Grid& g = ctx->g;
+    const int64_t count = g.n * (int64_t)ppc;
+    if (sp.n + count > sp.cap) {
+        ctx->err = "tmpg_load_uniform: capacity " + std::to_string(sp.cap) + " < " + std::to_string(sp.n + count);
+        return TMPG_EOVERFLOW;
+    }
+    const int ntab = 4096;
+    std::vector<float> tab = mj_inverse_cdf(theta, ntab);
+    float* dtab = nullptr;
+    TMPG_HIP_CHECK(ctx, hipMalloc(&dtab, ntab * sizeof(float)));
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(dtab, tab.data(), ntab * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    dim3 grid((unsigned)((count + 255) / 256));
+    hipLaunchKernelGGL(k_load, grid, dim3(256), 0, ctx->stream, sp.cell, sp.ox, sp.oy, sp.oz, sp.u, sp.v, sp.w,
+                       sp.id, sp.n, count, ppc, g.dim, g.nx, g.ny, ctx->pk, drift, dtab, ntab,
+                       seed * 2 + (uint64_t)s, (int32_t)sp.n);
+    hipError_t e = hipGetLastError();
+    hipStreamSynchronize(ctx->stream);
+    hipFree(dtab);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return TMPG_EHIP; }
+    sp.n += count;
+    return TMPG_OK;
+}
+
+}  // namespace tmpg

@@ -1,0 +1,271 @@
+// fields.hip -- Yee stencils, current filter and primal averaging on the
+// periodic interior grid (HBM-bound stencils).
+//
+// B half step: code/fields.F90:586-728 (advance_b_halfstep)
+// E step     : code/fields.F90:739-870 (advance_e_fullstep), optionally fused
+//              with add_current (code/fields.F90:1372-1395): E = (E + c*curl B) + J,
+//              the reference's two roundings in its order
+// filter     : code/filter.F90:8-221, (1,2,1)^d / 4^d per pass, periodic
+// The reference fills ghost layers (bc_b1/bc_e1) and folds ghost currents;
+// here every neighbour index wraps periodically instead.
+#include "tmpg_internal.h"
+
+namespace tmpg {
+namespace {
+
+constexpr int BX_ = 64, BY_ = 4;   // block: 64 x 4 threads, one x-row segment per wave
+
+__device__ __forceinline__ int wrapm(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_bhalf(float* __restrict__ ex, float* __restrict__ ey,
+                                               float* __restrict__ ez, float* __restrict__ bx,
+                                               float* __restrict__ by, float* __restrict__ bz,
+                                               int nx, int ny, int nz, float cst)
+{
+    int i = blockIdx.x * BX_ + threadIdx.x;
+    int j = blockIdx.y * BY_ + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    int64_t o = i + (int64_t)nx * (j + (int64_t)ny * k);
+    int ip = (i + 1 == nx) ? 0 : i + 1;
+    int jp = (j + 1 == ny) ? 0 : j + 1;
+    int64_t oi = ip + (int64_t)nx * (j + (int64_t)ny * k);
+    int64_t oj = i + (int64_t)nx * (jp + (int64_t)ny * k);
+    float exo = ex[o], eyo = ey[o], ezo = ez[o];
+    if (DIM == 3) {
+        int kp = (k + 1 == nz) ? 0 : k + 1;
+        int64_t ok = i + (int64_t)nx * (j + (int64_t)ny * kp);
+        bx[o] = bx[o] + cst * (ey[ok] - eyo - ez[oj] + ezo);
+        by[o] = by[o] + cst * (ez[oi] - ezo - ex[ok] + exo);
+        bz[o] = bz[o] + cst * (ex[oj] - exo - ey[oi] + eyo);
+    } else {
+        bx[o] = bx[o] + cst * (-ez[oj] + ezo);
+        by[o] = by[o] + cst * (ez[oi] - ezo);
+        bz[o] = bz[o] + cst * (ex[oj] - exo - ey[oi] + eyo);
+    }
+}
+
+template <int DIM, bool ADDJ>
+__global__ void __launch_bounds__(256) k_efull(float* __restrict__ ex, float* __restrict__ ey,
+                                               float* __restrict__ ez, const float* __restrict__ bx,
+                                               const float* __restrict__ by, const float* __restrict__ bz,
+                                               const float* __restrict__ jx, const float* __restrict__ jy,
+                                               const float* __restrict__ jz, int nx, int ny, int nz,
+                                               float cst)
+{
+    int i = blockIdx.x * BX_ + threadIdx.x;
+    int j = blockIdx.y * BY_ + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    int64_t o = i + (int64_t)nx * (j + (int64_t)ny * k);
+    int im = (i == 0) ? nx - 1 : i - 1;
+    int jm = (j == 0) ? ny - 1 : j - 1;
+    int64_t oi = im + (int64_t)nx * (j + (int64_t)ny * k);
+    int64_t oj = i + (int64_t)nx * (jm + (int64_t)ny * k);
+    float bxo = bx[o], byo = by[o], bzo = bz[o];
+    float nex, ney, nez;
+    if (DIM == 3) {
+        int km = (k == 0) ? nz - 1 : k - 1;
+        int64_t ok = i + (int64_t)nx * (j + (int64_t)ny * km);
+        nex = ex[o] + cst * (by[ok] - byo - bz[oj] + bzo);
+        ney = ey[o] + cst * (bz[oi] - bzo - bx[ok] + bxo);
+        nez = ez[o] + cst * (bx[oj] - bxo - by[oi] + byo);
+    } else {
+        nex = ex[o] + cst * (-bz[oj] + bzo);
+        ney = ey[o] + cst * (bz[oi] - bzo);
+        nez = ez[o] + cst * (bx[oj] - bxo - by[oi] + byo);
+    }
+    if (ADDJ) {
+        nex = nex + jx[o];
+        ney = ney + jy[o];
+        nez = nez + jz[o];
+    }
+    ex[o] = nex;
+    ey[o] = ney;
+    ez[o] = nez;
+}
+
+__global__ void k_add(float* __restrict__ e, const float* __restrict__ j, int64_t n)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) e[t] = e[t] + j[t];
+}
+
+// One binomial pass of one component: out = (1,2,1)^d/4^d * in, periodic.
+// Tile of 64 x 4 (x, y) cells per block, staged with a 1-cell halo in LDS for
+// the three z planes k-1, k, k+1; each output reads its 27 (9 in 2D)
+// neighbours from LDS.  The sum is the reference's (code/filter.F90:105-120).
+template <int DIM>
+__global__ void __launch_bounds__(256) k_filter(const float* __restrict__ in, float* __restrict__ out,
+                                                int nx, int ny, int nz)
+{
+    constexpr int NP = (DIM == 3) ? 3 : 1;
+    __shared__ float s[NP][BY_ + 2][BX_ + 2];
+    const int i0 = blockIdx.x * BX_, j0 = blockIdx.y * BY_, k = blockIdx.z;
+    const int tid = threadIdx.y * BX_ + threadIdx.x;
+    const int NT = BX_ * BY_;
+    constexpr int PLANE = (BY_ + 2) * (BX_ + 2);
+    for (int e = tid; e < NP * PLANE; e += NT) {
+        int p = e / PLANE, r = e - p * PLANE;
+        int ly = r / (BX_ + 2), lx = r - ly * (BX_ + 2);
+        int gi = wrapm(i0 + lx - 1, nx), gj = wrapm(j0 + ly - 1, ny);
+        int gk = (DIM == 3) ? wrapm(k + p - 1, nz) : 0;
+        s[p][ly][lx] = in[gi + (int64_t)nx * (gj + (int64_t)ny * gk)];
+    }
+    __syncthreads();
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const int x = threadIdx.x + 1, y = threadIdx.y + 1;
+    constexpr int c = (DIM == 3) ? 1 : 0;
+    float t;
+    if (DIM == 2) {
+        const float w = 1.0f / 16.0f;
+        const float wl = 2.0f * w, wt = 4.0f * w, wd = w;
+        t = wl * s[c][y][x - 1] + wt * s[c][y][x] + wl * s[c][y][x + 1] + wl * s[c][y - 1][x] +
+            wl * s[c][y + 1][x] + wd * s[c][y + 1][x - 1] + wd * s[c][y + 1][x + 1] +
+            wd * s[c][y - 1][x - 1] + wd * s[c][y - 1][x + 1];
+    } else {
+        const float w = 1.0f / 64.0f;
+        const float a = 4.0f * w, b = 8.0f * w, d = 2.0f * w, e = w;
+        t = a * s[1][y][x - 1] + b * s[1][y][x] + a * s[1][y][x + 1] + a * s[1][y - 1][x] +
+            a * s[1][y + 1][x] + d * s[1][y + 1][x - 1] + d * s[1][y + 1][x + 1] +
+            d * s[1][y - 1][x - 1] + d * s[1][y - 1][x + 1];
+        t = t + d * s[0][y][x - 1] + a * s[0][y][x] + d * s[0][y][x + 1] + d * s[0][y - 1][x] +
+            d * s[0][y + 1][x] + e * s[0][y + 1][x - 1] + e * s[0][y + 1][x + 1] +
+            e * s[0][y - 1][x - 1] + e * s[0][y - 1][x + 1] + d * s[2][y][x - 1] + a * s[2][y][x] +
+            d * s[2][y][x + 1] + d * s[2][y - 1][x] + d * s[2][y + 1][x] + e * s[2][y + 1][x - 1] +
+            e * s[2][y + 1][x + 1] + e * s[2][y - 1][x - 1] + e * s[2][y - 1][x + 1];
+    }
+    out[i + (int64_t)nx * (j + (int64_t)ny * k)] = t;
+}
+
+// primal-averaged fields for the 3D gather
+// (code/particles_movedeposit.F90:982-991): Ex, Ey, Ez averaged along their
+// own axis, Bx over y (and z unless Q_BXY_Z), By over x (and z), Bz over x,y.
+__global__ void __launch_bounds__(256) k_primal(const float* __restrict__ ex, const float* __restrict__ ey,
+                                                const float* __restrict__ ez, const float* __restrict__ bx,
+                                                const float* __restrict__ by, const float* __restrict__ bz,
+                                                float* __restrict__ pex, float* __restrict__ pey,
+                                                float* __restrict__ pez, float* __restrict__ pbx,
+                                                float* __restrict__ pby, float* __restrict__ pbz,
+                                                int nx, int ny, int nz, int quirk_z)
+{
+    int i = blockIdx.x * BX_ + threadIdx.x;
+    int j = blockIdx.y * BY_ + threadIdx.y;
+    int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    int im = (i == 0) ? nx - 1 : i - 1;
+    int jm = (j == 0) ? ny - 1 : j - 1;
+    int km = (k == 0) ? nz - 1 : k - 1;
+    auto I = [&](int a, int b, int c) { return a + (int64_t)nx * (b + (int64_t)ny * c); };
+    int64_t o = I(i, j, k);
+    pex[o] = 0.5f * (ex[o] + ex[I(im, j, k)]);
+    pey[o] = 0.5f * (ey[o] + ey[I(i, jm, k)]);
+    pez[o] = 0.5f * (ez[o] + ez[I(i, j, km)]);
+    if (quirk_z) {
+        pbx[o] = 0.5f * (bx[o] + bx[I(i, jm, k)]);
+        pby[o] = 0.5f * (by[o] + by[I(im, j, k)]);
+    } else {
+        pbx[o] = 0.5f * (0.5f * (bx[o] + bx[I(i, jm, k)]) + 0.5f * (bx[I(i, j, km)] + bx[I(i, jm, km)]));
+        pby[o] = 0.5f * (0.5f * (by[o] + by[I(im, j, k)]) + 0.5f * (by[I(i, j, km)] + by[I(im, j, km)]));
+    }
+    pbz[o] = 0.5f * (0.5f * (bz[o] + bz[I(im, j, k)]) + 0.5f * (bz[I(i, jm, k)] + bz[I(im, jm, k)]));
+}
+
+__global__ void k_zero(float* __restrict__ a, int64_t n)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) a[t] = 0.0f;
+}
+
+dim3 grid_of(const Grid& g) { return dim3((g.nx + BX_ - 1) / BX_, (g.ny + BY_ - 1) / BY_, g.nz); }
+
+}  // namespace
+
+int fields_bhalf(tmpg_ctx* ctx)
+{
+    const Grid& g = ctx->g;
+    float cst = ctx->cfg.corr * (0.5f * ctx->cfg.c);
+    float** f = ctx->f;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_bhalf<3>, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, f[EX], f[EY], f[EZ], f[BX],
+                           f[BY], f[BZ], g.nx, g.ny, g.nz, cst);
+    else
+        hipLaunchKernelGGL(k_bhalf<2>, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, f[EX], f[EY], f[EZ], f[BX],
+                           f[BY], f[BZ], g.nx, g.ny, g.nz, cst);
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    return TMPG_OK;
+}
+
+int fields_efull(tmpg_ctx* ctx, bool add_j)
+{
+    const Grid& g = ctx->g;
+    float cst = ctx->cfg.corr * ctx->cfg.c;
+    float** f = ctx->f;
+#define TMPG_EF(D, A)                                                                                   \
+    hipLaunchKernelGGL((k_efull<D, A>), grid_of(g), dim3(BX_, BY_), 0, ctx->stream, f[EX], f[EY], f[EZ], \
+                       f[BX], f[BY], f[BZ], f[JX], f[JY], f[JZ], g.nx, g.ny, g.nz, cst)
+    if (g.dim == 3) { if (add_j) TMPG_EF(3, true); else TMPG_EF(3, false); }
+    else { if (add_j) TMPG_EF(2, true); else TMPG_EF(2, false); }
+#undef TMPG_EF
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    return TMPG_OK;
+}
+
+int fields_add_current(tmpg_ctx* ctx)
+{
+    const Grid& g = ctx->g;
+    int blocks = (int)std::min<int64_t>((g.n + 255) / 256, 8192);
+    for (int q = 0; q < 3; ++q)
+        hipLaunchKernelGGL(k_add, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[EX + q], ctx->f[JX + q], g.n);
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    return TMPG_OK;
+}
+
+int fields_filter(tmpg_ctx* ctx, int npasses)
+{
+    const Grid& g = ctx->g;
+    for (int p = 0; p < npasses; ++p) {
+        for (int q = 0; q < 3; ++q) {
+            float* a = ctx->f[JX + q];
+            if (g.dim == 3)
+                hipLaunchKernelGGL(k_filter<3>, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, a, ctx->tmp, g.nx,
+                                   g.ny, g.nz);
+            else
+                hipLaunchKernelGGL(k_filter<2>, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, a, ctx->tmp, g.nx,
+                                   g.ny, g.nz);
+            // ping-pong: the filtered array becomes the current, the old one the scratch
+            ctx->f[JX + q] = ctx->tmp;
+            ctx->tmp = a;
+        }
+    }
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    return TMPG_OK;
+}
+
+int fields_primal(tmpg_ctx* ctx)
+{
+    const Grid& g = ctx->g;
+    float** f = ctx->f;
+    float** p = ctx->prim;
+    hipLaunchKernelGGL(k_primal, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, f[EX], f[EY], f[EZ], f[BX], f[BY],
+                       f[BZ], p[0], p[1], p[2], p[3], p[4], p[5], g.nx, g.ny, g.nz,
+                       (ctx->cfg.compat & TMPG_Q_BXY_Z) ? 1 : 0);
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    return TMPG_OK;
+}
+
+int fields_zero_j(tmpg_ctx* ctx)
+{
+    const Grid& g = ctx->g;
+    int blocks = (int)std::min<int64_t>((g.n + 255) / 256, 8192);
+    for (int q = 0; q < 3; ++q)
+        hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[JX + q], g.n);
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    return TMPG_OK;
+}
+
+}  // namespace tmpg

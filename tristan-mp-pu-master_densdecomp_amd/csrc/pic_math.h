@@ -1,0 +1,259 @@
+// pic_math.h -- per-particle arithmetic of the push and the deposit (FP32).
+//
+// Positions are an integer cell plus an offset o in [0,1).  A node n of a
+// shape window is the cell (particle cell + n).  All weights are the
+// reference's B-splines (code/particles_movedeposit.F90:447-468, 709-790,
+// 1035-1128; code/particles.F90:738-773, 925-1020, 1172-1277) written in
+// local polynomial pieces:
+//   order 1: piece X in [m, m+1)     nodes m, m+1        (knots at integers)
+//   order 2: piece X in [m-1/2,m+1/2) nodes m-1, m, m+1  (knots at half-integers)
+//   order 3: piece X in [m, m+1)     nodes m-1 .. m+2    (knots at integers)
+// The Esirkepov deposit needs dS = S(new) - S(old).  Instead of subtracting
+// two O(1) numbers, dS is evaluated as (length) x (divided difference) on
+// each polynomial piece the move crosses (at most one knot, since
+// |displacement| < 1 cell), which keeps its relative error at FP32 round-off
+// even for slow particles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tmpg {
+
+template <int ORD> struct Win;                 // deposit window per axis
+template <> struct Win<1> { static constexpr int W = 3; };
+template <> struct Win<2> { static constexpr int W = 4; };
+template <> struct Win<3> { static constexpr int W = 5; };
+
+// ---------------------------------------------------------------- weights
+// node weights of a particle at local coordinate xi of piece m; writes the
+// weights of nodes (m + first .. m + first + cnt - 1) into w[0..cnt-1].
+template <int ORD> struct Piece;
+
+template <> struct Piece<1> {
+    static constexpr int first = 0, cnt = 2;
+    __device__ __forceinline__ static void w(float xi, float* o) {
+        o[0] = 1.0f - xi;
+        o[1] = xi;
+    }
+    // dS over a segment [a,b] inside the piece, d = b - a computed accurately
+    __device__ __forceinline__ static void dw(float a, float b, float d, float* o) {
+        (void)a; (void)b;
+        o[0] = -d;
+        o[1] = d;
+    }
+};
+
+template <> struct Piece<2> {
+    static constexpr int first = -1, cnt = 3;
+    __device__ __forceinline__ static void w(float xi, float* o) {
+        float l = 0.5f - xi, r = 0.5f + xi;
+        o[0] = 0.5f * l * l;
+        o[1] = 0.75f - xi * xi;
+        o[2] = 0.5f * r * r;
+    }
+    __device__ __forceinline__ static void dw(float a, float b, float d, float* o) {
+        float s = a + b;
+        o[0] = -0.5f * d * (1.0f - s);
+        o[1] = -d * s;
+        o[2] = 0.5f * d * (1.0f + s);
+    }
+};
+
+template <> struct Piece<3> {
+    static constexpr int first = -1, cnt = 4;
+    __device__ __forceinline__ static void w(float xi, float* o) {
+        float r = 1.0f - xi;
+        const float sixth = 1.0f / 6.0f, twoth = 2.0f / 3.0f;
+        o[0] = sixth * r * r * r;
+        o[1] = twoth - xi * xi + 0.5f * xi * xi * xi;
+        o[2] = twoth - r * r + 0.5f * r * r * r;
+        o[3] = sixth * xi * xi * xi;
+    }
+    __device__ __forceinline__ static void dw(float a, float b, float d, float* o) {
+        const float sixth = 1.0f / 6.0f;
+        float A = 1.0f - a, B = 1.0f - b;
+        float q = a * a + a * b + b * b;
+        float Q = A * A + A * B + B * B;
+        o[0] = -d * Q * sixth;
+        o[1] = d * (0.5f * q - (a + b));
+        o[2] = -d * (0.5f * Q - (A + B));
+        o[3] = d * q * sixth;
+    }
+};
+
+// piece containing the local position X (relative to the particle's cell).
+// Order 2 pieces are split at half-integers and X = m + 1/2 belongs to the
+// lower piece, as in the reference's `d <= half` branch.
+template <int ORD> __device__ __forceinline__ int piece_of(float X)
+{
+    return (ORD == 2) ? (int)ceilf(X - 0.5f) : (int)floorf(X);
+}
+// the knot between two adjacent pieces ma < mb
+template <int ORD> __device__ __forceinline__ float knot_between(int ma, int mb)
+{
+    int m = ma > mb ? ma : mb;
+    return (ORD == 2) ? (float)m - 0.5f : (float)m;
+}
+
+// -------------------------------------------------- one axis of a deposit
+// Given old offset o in [0,1) and displacement dl (|dl| < 1), fill, on the
+// window of Win<ORD>::W nodes starting at node `base` (relative to the old
+// cell):  S1 = old weights, dS = new - old weights.  Returns base.
+template <int ORD>
+__device__ __forceinline__ int axis_deposit(float o, float dl, float* S1, float* dS)
+{
+    constexpr int W = Win<ORD>::W;
+    using P = Piece<ORD>;
+    const float X1 = o + dl;
+    const int m0 = piece_of<ORD>(o);
+    const int m1 = piece_of<ORD>(X1);
+    // window: from the first node of the lower piece; W nodes hold both
+    // supports because the two pieces are adjacent
+    const int base = (m0 < m1 ? m0 : m1) + P::first;
+#pragma unroll
+    for (int k = 0; k < W; ++k) { S1[k] = 0.0f; dS[k] = 0.0f; }
+    float t[P::cnt];
+    // old weights
+    {
+        float xi = o - (float)m0;  // exact: o and m0 within a factor 2 or m0 = 0
+        P::w(xi, t);
+        const int off = m0 + P::first - base;
+#pragma unroll
+        for (int k = 0; k < P::cnt; ++k) {
+#pragma unroll
+            for (int q = 0; q < W; ++q)
+                if (q == off + k) S1[q] = t[k];
+        }
+    }
+    if (m0 == m1) {
+        float a = o - (float)m0;
+        float b = a + dl;
+        P::dw(a, b, dl, t);
+        const int off = m0 + P::first - base;
+#pragma unroll
+        for (int k = 0; k < P::cnt; ++k)
+#pragma unroll
+            for (int q = 0; q < W; ++q)
+                if (q == off + k) dS[q] += t[k];
+    } else {
+        // knot between the two pieces
+        const float K = knot_between<ORD>(m0, m1);
+        const float d1 = K - o;            // first segment length
+        const float d2 = dl - d1;          // second segment length
+        {
+            float a = o - (float)m0;
+            float b = K - (float)m0;
+            P::dw(a, b, d1, t);
+            const int off = m0 + P::first - base;
+#pragma unroll
+            for (int k = 0; k < P::cnt; ++k)
+#pragma unroll
+                for (int q = 0; q < W; ++q)
+                    if (q == off + k) dS[q] += t[k];
+        }
+        {
+            float a = K - (float)m1;
+            float b = a + d2;
+            P::dw(a, b, d2, t);
+            const int off = m1 + P::first - base;
+#pragma unroll
+            for (int k = 0; k < P::cnt; ++k)
+#pragma unroll
+                for (int q = 0; q < W; ++q)
+                    if (q == off + k) dS[q] += t[k];
+        }
+    }
+    return base;
+}
+
+// ------------------------------------------------------- gather weights
+// Weights of nodes -1..2 relative to cell i for a particle at offset o,
+// primal (Sp, at the cell nodes) and dual (Sd, relative to cell id = i or
+// i-1, at offset o -+ 1/2), as the movers compute them.  For order 2 with
+// `quirk` the primal weights are masked to the dual's slot range, as the
+// reference's loops do (code/particles_movedeposit.F90:709-832).
+template <int ORD>
+__device__ __forceinline__ void gather_weights(float o, bool quirk, float* Sp, float* Sd, int& did)
+{
+    using P = Piece<ORD>;
+    float t[P::cnt];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { Sp[k] = 0.0f; Sd[k] = 0.0f; }
+    // primal
+    const int mp = piece_of<ORD>(o);
+    P::w(o - (float)mp, t);
+#pragma unroll
+    for (int k = 0; k < P::cnt; ++k)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q == mp + P::first + k + 1) Sp[q] = t[k];
+    // dual: id = i if o >= 1/2 else i - 1
+    float od;
+    if (o >= 0.5f) { did = 0; od = o - 0.5f; } else { did = -1; od = o + 0.5f; }
+    const int md = piece_of<ORD>(od);
+    P::w(od - (float)md, t);
+#pragma unroll
+    for (int k = 0; k < P::cnt; ++k)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q == md + P::first + k + 1) Sd[q] = t[k];
+    if (ORD == 2 && quirk) {
+        // the reference loops over the dual's slots: nodes (md-1 .. md+1)
+        // of the 6-slot frame, applied with the primal's cell origin
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int node = k - 1;
+            if (node < md - 1 || node > md + 1) Sp[k] = 0.0f;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Boris
+// code/particles_movedeposit.F90:1172-1177, 1224-1267.  e, b are the
+// gathered fields already scaled: e = 0.5*qm*E, b = 0.5*qm*B/c.  u in units
+// of c (gamma*beta).  Returns the displacement in cells.
+__device__ __forceinline__ void boris(float c, float cinv, float ex0, float ey0, float ez0,
+                                      float bx0, float by0, float bz0,
+                                      float& u, float& v, float& w,
+                                      float& dx, float& dy, float& dz)
+{
+    float u0 = c * u + ex0;
+    float v0 = c * v + ey0;
+    float w0 = c * w + ez0;
+    float g = c * __frsqrt_rn(c * c + u0 * u0 + v0 * v0 + w0 * w0);
+    bx0 *= g; by0 *= g; bz0 *= g;
+    float f = 2.0f / (1.0f + bx0 * bx0 + by0 * by0 + bz0 * bz0);
+    float u1 = (u0 + v0 * bz0 - w0 * by0) * f;
+    float v1 = (v0 + w0 * bx0 - u0 * bz0) * f;
+    float w1 = (w0 + u0 * by0 - v0 * bx0) * f;
+    u0 = u0 + v1 * bz0 - w1 * by0 + ex0;
+    v0 = v0 + w1 * bx0 - u1 * bz0 + ey0;
+    w0 = w0 + u1 * by0 - v1 * bx0 + ez0;
+    u = u0 * cinv;
+    v = v0 * cinv;
+    w = w0 * cinv;
+    g = c * __frsqrt_rn(c * c + u0 * u0 + v0 * v0 + w0 * w0);
+    dx = u * g * c;
+    dy = v * g * c;
+    dz = w * g * c;
+}
+
+// advance offset o by dl; returns the cell shift (-1, 0, +1) and the new
+// offset in [0,1)
+__device__ __forceinline__ int advance_offset(float& o, float dl)
+{
+    float t = o + dl;
+    if (t >= 1.0f) {
+        float r = dl - (1.0f - o);
+        o = r < 0.0f ? 0.0f : r;
+        return 1;
+    }
+    if (t < 0.0f) {
+        float r = 1.0f - ((-dl) - o);
+        o = r >= 1.0f ? 0.99999994f : r;
+        return -1;
+    }
+    o = t;
+    return 0;
+}
+
+}  // namespace tmpg

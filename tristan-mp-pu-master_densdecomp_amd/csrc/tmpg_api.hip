@@ -1,0 +1,680 @@
+// tmpg_api.hip -- the C ABI of include/tmpg.h: context, boundary layout
+// conversions, the step sequence and the input-file reader.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "tmpg_internal.h"
+
+using namespace tmpg;
+
+namespace {
+
+std::string g_create_err;
+
+int bits_for(int n)
+{
+    int b = 0;
+    while ((1 << b) < n) ++b;
+    return b;
+}
+
+template <typename T>
+int dalloc(tmpg_ctx* ctx, T** p, int64_t n)
+{
+    *p = nullptr;
+    if (n <= 0) return TMPG_OK;
+    hipError_t e = hipMalloc((void**)p, (size_t)n * sizeof(T));
+    if (e != hipSuccess) {
+        ctx->err = "hipMalloc(" + std::to_string((size_t)n * sizeof(T)) + " B): " + hipGetErrorString(e);
+        *p = nullptr;
+        return TMPG_ENOMEM;
+    }
+    return TMPG_OK;
+}
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+void free_species(Species& s)
+{
+    void* ps[] = {s.cell, s.ox, s.oy, s.oz, s.u, s.v, s.w, s.ch, s.id, s.cell2, s.ox2, s.oy2, s.oz2, s.u2, s.v2,
+                  s.w2, s.ch2, s.id2, s.cnt, s.cnt2, s.cell_old, s.ox_old, s.oy_old, s.oz_old, s.dx, s.dy, s.dz};
+    for (void* p : ps) dfree(p);
+    s = Species();
+}
+
+int alloc_species(tmpg_ctx* ctx, Species& s, int64_t cap, bool has_ch, bool ids)
+{
+    const bool d3 = ctx->g.dim == 3;
+    int rc = 0;
+    s.cap = cap;
+    s.has_ch = has_ch;
+    rc |= dalloc(ctx, &s.cell, cap);
+    rc |= dalloc(ctx, &s.ox, cap);
+    rc |= dalloc(ctx, &s.oy, cap);
+    if (d3) rc |= dalloc(ctx, &s.oz, cap);
+    rc |= dalloc(ctx, &s.u, cap);
+    rc |= dalloc(ctx, &s.v, cap);
+    rc |= dalloc(ctx, &s.w, cap);
+    if (has_ch) rc |= dalloc(ctx, &s.ch, cap);
+    if (ids) rc |= dalloc(ctx, &s.id, cap);
+    return rc ? TMPG_ENOMEM : TMPG_OK;
+}
+
+int alloc_split_scratch(tmpg_ctx* ctx, Species& s)
+{
+    if (s.dx) return TMPG_OK;
+    const int64_t cap = s.cap;
+    int rc = 0;
+    rc |= dalloc(ctx, &s.cell_old, cap);
+    rc |= dalloc(ctx, &s.ox_old, cap);
+    rc |= dalloc(ctx, &s.oy_old, cap);
+    if (ctx->g.dim == 3) rc |= dalloc(ctx, &s.oz_old, cap);
+    rc |= dalloc(ctx, &s.dx, cap);
+    rc |= dalloc(ctx, &s.dy, cap);
+    rc |= dalloc(ctx, &s.dz, cap);
+    return rc ? TMPG_ENOMEM : TMPG_OK;
+}
+
+// boundary (Fortran padded, ghosts included) <-> interior
+void pad_to_interior(const tmpg_ctx* ctx, const float* src, std::vector<float>& dst)
+{
+    const Grid& g = ctx->g;
+    dst.resize((size_t)g.n);
+    const int k0 = (g.dim == 3) ? ctx->hz : 0;
+    for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j) {
+            const float* s = src + (size_t)(ctx->h) + (size_t)ctx->mx * ((size_t)(j + ctx->h) + (size_t)ctx->my * (k + k0));
+            std::memcpy(&dst[(size_t)g.nx * (j + (size_t)g.ny * k)], s, (size_t)g.nx * sizeof(float));
+        }
+}
+
+// interior -> padded array; every ghost cell holds its periodic image
+void interior_to_pad(const tmpg_ctx* ctx, const std::vector<float>& src, float* dst)
+{
+    const Grid& g = ctx->g;
+    const int k0 = (g.dim == 3) ? ctx->hz : 0;
+    auto wrap = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };
+    for (int kk = 0; kk < ctx->mz; ++kk) {
+        int k = (g.dim == 3) ? wrap(kk - k0, g.nz) : 0;
+        for (int jj = 0; jj < ctx->my; ++jj) {
+            int j = wrap(jj - ctx->h, g.ny);
+            const float* row = &src[(size_t)g.nx * (j + (size_t)g.ny * k)];
+            float* d = dst + (size_t)ctx->mx * (jj + (size_t)ctx->my * kk);
+            for (int ii = 0; ii < ctx->mx; ++ii) d[ii] = row[wrap(ii - ctx->h, g.nx)];
+        }
+    }
+}
+
+int set_grid(tmpg_ctx* ctx, int first, int count, const float* const* arrays)
+{
+    std::vector<float> buf;
+    for (int q = 0; q < count; ++q) {
+        if (!arrays[q]) { ctx->err = "null array"; return TMPG_EINVAL; }
+        pad_to_interior(ctx, arrays[q], buf);
+        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(ctx->f[first + q], buf.data(), buf.size() * sizeof(float),
+                                           hipMemcpyHostToDevice, ctx->stream));
+        TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return TMPG_OK;
+}
+
+int get_grid(tmpg_ctx* ctx, int first, int count, float* const* arrays)
+{
+    std::vector<float> buf((size_t)ctx->g.n);
+    for (int q = 0; q < count; ++q) {
+        if (!arrays[q]) { ctx->err = "null array"; return TMPG_EINVAL; }
+        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(buf.data(), ctx->f[first + q], buf.size() * sizeof(float),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+        TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        interior_to_pad(ctx, buf, arrays[q]);
+    }
+    return TMPG_OK;
+}
+
+bool valid_ctx(const tmpg_ctx* ctx) { return ctx != nullptr; }
+
+}  // namespace
+
+extern "C" {
+
+const char* tmpg_last_error(const tmpg_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
+{
+    if (!cfg || !out) { g_create_err = "null argument"; return TMPG_EINVAL; }
+    *out = nullptr;
+    if (cfg->dim != 2 && cfg->dim != 3) { g_create_err = "dim must be 2 or 3"; return TMPG_EINVAL; }
+    if (cfg->order < 1 || cfg->order > 3) { g_create_err = "order must be 1, 2 or 3"; return TMPG_EINVAL; }
+    const int nz = cfg->dim == 3 ? cfg->nz : 1;
+    if (cfg->nx < 4 || cfg->ny < 4 || nz < 1 || (cfg->dim == 3 && nz < 4)) {
+        g_create_err = "grid too small (need at least 4 cells per periodic axis)";
+        return TMPG_EINVAL;
+    }
+    const int bx = bits_for(cfg->nx), by = bits_for(cfg->ny), bz = bits_for(nz);
+    if (bx + by + bz > 31) { g_create_err = "grid too large for the 31-bit packed cell index"; return TMPG_EINVAL; }
+    if (!(cfg->c > 0.0f && cfg->c < 1.0f)) { g_create_err = "c must be in (0,1)"; return TMPG_EINVAL; }
+    tmpg_ctx* ctx = new (std::nothrow) tmpg_ctx();
+    if (!ctx) { g_create_err = "out of host memory"; return TMPG_ENOMEM; }
+    ctx->cfg = *cfg;
+    ctx->cfg.nz = nz;
+    ctx->g.dim = cfg->dim;
+    ctx->g.nx = cfg->nx;
+    ctx->g.ny = cfg->ny;
+    ctx->g.nz = nz;
+    ctx->g.n = (int64_t)cfg->nx * cfg->ny * nz;
+    ctx->pk.bx = bx;
+    ctx->pk.by = by;
+    ctx->pk.mask_x = (1 << bx) - 1;
+    ctx->pk.mask_y = (1 << by) - 1;
+    const int ng = cfg->order == 1 ? 5 : 7;
+    const int ngz = cfg->dim == 2 ? 5 : ng;
+    ctx->h = ng / 2;
+    ctx->hz = ngz / 2;
+    ctx->mx = cfg->nx + ng;
+    ctx->my = cfg->ny + ng;
+    ctx->mz = cfg->dim == 3 ? nz + ngz : 1;
+    ctx->tl = Tiling{ctx->g.nx, ctx->g.ny, ctx->g.nz, 1, 1, 1, 1};
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        delete ctx;
+        return TMPG_EHIP;
+    }
+    int rc = 0;
+    for (int q = 0; q < NF; ++q) rc |= dalloc(ctx, &ctx->f[q], ctx->g.n);
+    rc |= dalloc(ctx, &ctx->tmp, ctx->g.n);
+    if (cfg->dim == 3)
+        for (int q = 0; q < 6; ++q) rc |= dalloc(ctx, &ctx->prim[q], ctx->g.n);
+    rc |= dalloc(ctx, &ctx->red, 16);
+    rc |= dalloc(ctx, &ctx->flags, 16);
+    if (rc) {
+        g_create_err = ctx->err;
+        tmpg_destroy(ctx);
+        return TMPG_ENOMEM;
+    }
+    for (int q = 0; q < NF; ++q) (void)hipMemsetAsync(ctx->f[q], 0, ctx->g.n * sizeof(float), ctx->stream);
+    (void)hipMemsetAsync(ctx->flags, 0, 16 * sizeof(int32_t), ctx->stream);
+    for (auto& ev : ctx->ev) (void)hipEventCreate(&ev);
+    for (int s = 0; s < 2; ++s) {
+        if (cfg->capacity[s] > 0) {
+            if (alloc_species(ctx, ctx->sp[s], cfg->capacity[s], false, cfg->track_ids != 0)) {
+                g_create_err = ctx->err;
+                tmpg_destroy(ctx);
+                return TMPG_ENOMEM;
+            }
+        }
+    }
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        g_create_err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+        tmpg_destroy(ctx);
+        return TMPG_EHIP;
+    }
+    *out = ctx;
+    return TMPG_OK;
+}
+
+int tmpg_destroy(tmpg_ctx* ctx)
+{
+    if (!ctx) return TMPG_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& p : ctx->f) dfree(p);
+    for (auto& p : ctx->prim) dfree(p);
+    dfree(ctx->tmp);
+    dfree(ctx->rho);
+    dfree(ctx->rho_prev);
+    dfree(ctx->red);
+    dfree(ctx->flags);
+    for (auto& s : ctx->sp) free_species(s);
+    for (auto& ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TMPG_OK;
+}
+
+int tmpg_sync(tmpg_ctx* ctx)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TMPG_OK;
+}
+
+int tmpg_set_fields(tmpg_ctx* ctx, const float* ex, const float* ey, const float* ez, const float* bx,
+                    const float* by, const float* bz)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    const float* a[6] = {ex, ey, ez, bx, by, bz};
+    return set_grid(ctx, EX, 6, a);
+}
+
+int tmpg_get_fields(tmpg_ctx* ctx, float* ex, float* ey, float* ez, float* bx, float* by, float* bz)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    float* a[6] = {ex, ey, ez, bx, by, bz};
+    return get_grid(ctx, EX, 6, a);
+}
+
+int tmpg_set_currents(tmpg_ctx* ctx, const float* jx, const float* jy, const float* jz)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    const float* a[3] = {jx, jy, jz};
+    return set_grid(ctx, JX, 3, a);
+}
+
+int tmpg_get_currents(tmpg_ctx* ctx, float* jx, float* jy, float* jz)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    float* a[3] = {jx, jy, jz};
+    return get_grid(ctx, JX, 3, a);
+}
+
+int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, const float* y, const float* z,
+                       const float* u, const float* v, const float* w, const float* ch, const int32_t* id)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    if (species < 0 || species > 1 || n < 0) { ctx->err = "bad species or count"; return TMPG_EINVAL; }
+    if (n > 0 && (!x || !y || !u || !v || !w || (ctx->g.dim == 3 && !z))) {
+        ctx->err = "null particle array";
+        return TMPG_EINVAL;
+    }
+    Species& sp = ctx->sp[species];
+    bool need_ch = false;
+    if (ch)
+        for (int64_t p = 0; p < n; ++p)
+            if (ch[p] != 1.0f) { need_ch = true; break; }
+    const bool ids = ctx->cfg.track_ids != 0;
+    if (n > sp.cap || (need_ch && !sp.ch)) {
+        int64_t cap = std::max<int64_t>(n, sp.cap);
+        free_species(sp);
+        int rc = alloc_species(ctx, sp, cap, need_ch, ids);
+        if (rc) return rc;
+    }
+    sp.has_ch = need_ch;
+    std::vector<int32_t> cell((size_t)n);
+    std::vector<float> o[3];
+    for (auto& a : o) a.resize((size_t)n);
+    const Grid& g = ctx->g;
+    const double base_xy = ctx->h + 1, base_z = ctx->hz + 1;
+    for (int64_t p = 0; p < n; ++p) {
+        double r[3] = {(double)x[p] - base_xy, (double)y[p] - base_xy, g.dim == 3 ? (double)z[p] - base_z : 0.0};
+        const int nn[3] = {g.nx, g.ny, g.nz};
+        int c[3];
+        for (int a = 0; a < 3; ++a) {
+            if (!std::isfinite(r[a])) { ctx->err = "non-finite particle coordinate"; return TMPG_EINVAL; }
+            double fl = std::floor(r[a]);
+            double off = r[a] - fl;
+            long long ci = (long long)fl;
+            ci %= nn[a];
+            if (ci < 0) ci += nn[a];
+            float of = (float)off;
+            if (of >= 1.0f) { of = 0.0f; ci = (ci + 1) % nn[a]; }
+            c[a] = (int)ci;
+            o[a][p] = of;
+        }
+        if (g.dim == 2) c[2] = 0;
+        cell[p] = pack_cell(ctx->pk, c[0], c[1], c[2]);
+    }
+    auto up = [&](void* d, const void* s, size_t bytes) -> int {
+        if (!bytes) return TMPG_OK;
+        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return TMPG_OK;
+    };
+    const size_t fb = (size_t)n * sizeof(float);
+    int rc = 0;
+    rc |= up(sp.cell, cell.data(), (size_t)n * sizeof(int32_t));
+    rc |= up(sp.ox, o[0].data(), fb);
+    rc |= up(sp.oy, o[1].data(), fb);
+    if (g.dim == 3) rc |= up(sp.oz, o[2].data(), fb);
+    rc |= up(sp.u, u, fb);
+    rc |= up(sp.v, v, fb);
+    rc |= up(sp.w, w, fb);
+    if (need_ch) rc |= up(sp.ch, ch, fb);
+    if (ids && sp.id) {
+        std::vector<int32_t> tmp;
+        const int32_t* src = id;
+        if (!src) {
+            tmp.resize((size_t)n);
+            for (int64_t p = 0; p < n; ++p) tmp[(size_t)p] = (int32_t)p;
+            src = tmp.data();
+        }
+        rc |= up(sp.id, src, (size_t)n * sizeof(int32_t));
+        TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (rc) return rc;
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    sp.n = n;
+    ctx->move_pending = false;
+    ctx->have_rho_prev = false;
+    return TMPG_OK;
+}
+
+int tmpg_count_particles(tmpg_ctx* ctx, int species, int64_t* n)
+{
+    if (!valid_ctx(ctx) || species < 0 || species > 1 || !n) return TMPG_EINVAL;
+    *n = ctx->sp[species].n;
+    return TMPG_OK;
+}
+
+int tmpg_get_particles(tmpg_ctx* ctx, int species, int64_t n_capacity, int64_t* n, float* x, float* y, float* z,
+                       float* u, float* v, float* w, float* ch, int32_t* id)
+{
+    if (!valid_ctx(ctx) || species < 0 || species > 1) return TMPG_EINVAL;
+    Species& sp = ctx->sp[species];
+    const int64_t m = std::min<int64_t>(n_capacity, sp.n);
+    if (n) *n = m;
+    if (m <= 0) return TMPG_OK;
+    const Grid& g = ctx->g;
+    std::vector<int32_t> cell((size_t)m);
+    std::vector<float> o[3];
+    for (auto& a : o) a.resize((size_t)m);
+    const size_t fb = (size_t)m * sizeof(float);
+    auto down = [&](void* d, const void* s, size_t bytes) -> int {
+        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        return TMPG_OK;
+    };
+    int rc = 0;
+    rc |= down(cell.data(), sp.cell, (size_t)m * sizeof(int32_t));
+    rc |= down(o[0].data(), sp.ox, fb);
+    rc |= down(o[1].data(), sp.oy, fb);
+    if (g.dim == 3) rc |= down(o[2].data(), sp.oz, fb);
+    if (u) rc |= down(u, sp.u, fb);
+    if (v) rc |= down(v, sp.v, fb);
+    if (w) rc |= down(w, sp.w, fb);
+    if (id && sp.id) rc |= down(id, sp.id, (size_t)m * sizeof(int32_t));
+    if (rc) return rc;
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ch) {
+        if (sp.has_ch) {
+            TMPG_HIP_CHECK(ctx, hipMemcpy(ch, sp.ch, fb, hipMemcpyDeviceToHost));
+        } else {
+            std::fill(ch, ch + m, 1.0f);
+        }
+    }
+    const double bxy = ctx->h + 1, bz = ctx->hz + 1;
+    for (int64_t p = 0; p < m; ++p) {
+        int i, j, k;
+        unpack_cell(ctx->pk, cell[(size_t)p], i, j, k);
+        if (x) x[p] = (float)(bxy + i + (double)o[0][(size_t)p]);
+        if (y) y[p] = (float)(bxy + j + (double)o[1][(size_t)p]);
+        if (z) z[p] = (float)(g.dim == 3 ? bz + k + (double)o[2][(size_t)p] : bz + 0.5);
+    }
+    return TMPG_OK;
+}
+
+int tmpg_advance_b_half(tmpg_ctx* ctx)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    return fields_bhalf(ctx);
+}
+
+int tmpg_advance_e(tmpg_ctx* ctx)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    return fields_efull(ctx, false);
+}
+
+int tmpg_reset_currents(tmpg_ctx* ctx)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    return fields_zero_j(ctx);
+}
+
+static int ensure_rho(tmpg_ctx* ctx)
+{
+    int rc = 0;
+    if (!ctx->rho) rc |= dalloc(ctx, &ctx->rho, ctx->g.n);
+    if (!ctx->rho_prev) rc |= dalloc(ctx, &ctx->rho_prev, ctx->g.n);
+    return rc ? TMPG_ENOMEM : TMPG_OK;
+}
+
+static int pre_push(tmpg_ctx* ctx)
+{
+    if (!ctx->track_cont) return TMPG_OK;
+    int rc = ensure_rho(ctx);
+    if (rc) return rc;
+    rc = diag_rho(ctx, ctx->rho_prev);
+    ctx->have_rho_prev = (rc == TMPG_OK);
+    return rc;
+}
+
+int tmpg_move(tmpg_ctx* ctx)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    for (auto& s : ctx->sp)
+        if (s.n > 0) {
+            int rc = alloc_split_scratch(ctx, s);
+            if (rc) return rc;
+        }
+    int rc = pre_push(ctx);
+    if (rc) return rc;
+    rc = particles_push(ctx, 1);
+    if (rc == TMPG_OK) ctx->move_pending = true;
+    return rc;
+}
+
+int tmpg_deposit(tmpg_ctx* ctx)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    if (!ctx->move_pending) { ctx->err = "tmpg_deposit: no tmpg_move to deposit"; return TMPG_ESTATE; }
+    int rc = particles_push(ctx, 2);
+    if (rc == TMPG_OK) ctx->move_pending = false;
+    return rc;
+}
+
+int tmpg_push_deposit(tmpg_ctx* ctx)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    int rc = pre_push(ctx);
+    if (rc) return rc;
+    ctx->move_pending = false;
+    return particles_push(ctx, 0);
+}
+
+int tmpg_fold_currents(tmpg_ctx* ctx) { return valid_ctx(ctx) ? TMPG_OK : TMPG_EINVAL; }
+
+int tmpg_filter(tmpg_ctx* ctx, int npasses)
+{
+    if (!valid_ctx(ctx) || npasses < 0) return TMPG_EINVAL;
+    return fields_filter(ctx, npasses);
+}
+
+int tmpg_add_current(tmpg_ctx* ctx)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    return fields_add_current(ctx);
+}
+
+int tmpg_step(tmpg_ctx* ctx, int nsteps)
+{
+    if (!valid_ctx(ctx) || nsteps < 0) return TMPG_EINVAL;
+    int rc = 0;
+    hipEvent_t* ev = ctx->ev;
+    for (int s = 0; s < nsteps && rc == 0; ++s) {
+        // code/tristanmainloop.F90:117-242 with the ghost fills and the
+        // current fold absorbed into periodic indexing, reset_currents moved
+        // before the push and add_current fused into the E update
+        (void)hipEventRecord(ev[0], ctx->stream);
+        rc = fields_bhalf(ctx);
+        if (!rc) rc = fields_zero_j(ctx);
+        (void)hipEventRecord(ev[1], ctx->stream);
+        if (!rc) rc = pre_push(ctx);
+        if (!rc) rc = particles_push(ctx, 0);
+        (void)hipEventRecord(ev[2], ctx->stream);
+        if (!rc) rc = fields_bhalf(ctx);
+        (void)hipEventRecord(ev[3], ctx->stream);
+        if (!rc) rc = fields_filter(ctx, ctx->cfg.ntimes);
+        (void)hipEventRecord(ev[4], ctx->stream);
+        if (!rc) rc = fields_efull(ctx, true);
+        (void)hipEventRecord(ev[5], ctx->stream);
+    }
+    ctx->move_pending = false;
+    return rc;
+}
+
+int tmpg_last_timings(tmpg_ctx* ctx, double ms[5])
+{
+    if (!valid_ctx(ctx) || !ms) return TMPG_EINVAL;
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    float t[5] = {0, 0, 0, 0, 0};
+    (void)hipEventElapsedTime(&t[0], ctx->ev[0], ctx->ev[1]);
+    float t2 = 0, t3 = 0;
+    (void)hipEventElapsedTime(&t[1], ctx->ev[1], ctx->ev[2]);
+    (void)hipEventElapsedTime(&t2, ctx->ev[2], ctx->ev[3]);
+    (void)hipEventElapsedTime(&t[2], ctx->ev[3], ctx->ev[4]);
+    (void)hipEventElapsedTime(&t3, ctx->ev[4], ctx->ev[5]);
+    (void)hipEventElapsedTime(&t[4], ctx->ev[0], ctx->ev[5]);
+    ms[0] = (double)t[0] + t2 + t3;
+    ms[1] = t[1];
+    ms[2] = t[2];
+    ms[3] = 0.0;
+    ms[4] = t[4];
+    return TMPG_OK;
+}
+
+int tmpg_track_continuity(tmpg_ctx* ctx, int on)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    ctx->track_cont = on != 0;
+    ctx->have_rho_prev = false;
+    return on ? ensure_rho(ctx) : TMPG_OK;
+}
+
+int tmpg_charge_density(tmpg_ctx* ctx, double* rho)
+{
+    if (!valid_ctx(ctx) || !rho) return TMPG_EINVAL;
+    int rc = ensure_rho(ctx);
+    if (rc) return rc;
+    rc = diag_rho(ctx, ctx->rho);
+    if (rc) return rc;
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(rho, ctx->rho, ctx->g.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TMPG_OK;
+}
+
+int tmpg_diag(tmpg_ctx* ctx, double out[8])
+{
+    if (!valid_ctx(ctx) || !out) return TMPG_EINVAL;
+    double e[4];
+    int rc = diag_energies(ctx, e);
+    if (rc) return rc;
+    out[0] = e[0];
+    out[1] = e[1];
+    out[2] = e[2];
+    out[3] = e[3];
+    out[4] = (double)ctx->sp[0].n;
+    out[5] = (double)ctx->sp[1].n;
+    out[6] = -1.0;
+    out[7] = 0.0;
+    if (ctx->track_cont && ctx->have_rho_prev) {
+        rc = diag_continuity(ctx, &out[6], &out[7]);
+        if (rc) return rc;
+    }
+    return TMPG_OK;
+}
+
+int tmpg_load_uniform(tmpg_ctx* ctx, int species, int ppc, float drift_gb, float theta, uint64_t seed)
+{
+    if (!valid_ctx(ctx) || species < 0 || species > 1 || ppc < 0 || theta < 0.0f) return TMPG_EINVAL;
+    Species& sp = ctx->sp[species];
+    if (sp.has_ch) { ctx->err = "tmpg_load_uniform: species has per-particle weights"; return TMPG_ESTATE; }
+    const int64_t need = sp.n + ctx->g.n * (int64_t)ppc;
+    if (need > sp.cap) {
+        if (sp.n > 0) {
+            ctx->err = "tmpg_load_uniform: capacity too small (set tmpg_config.capacity)";
+            return TMPG_EOVERFLOW;
+        }
+        free_species(sp);
+        int rc = alloc_species(ctx, sp, need, false, ctx->cfg.track_ids != 0);
+        if (rc) return rc;
+    }
+    ctx->have_rho_prev = false;
+    return particles_load_uniform(ctx, species, ppc, drift_gb, theta, seed);
+}
+
+// ------------------------------------------------------------------ input
+// The reference's input format (code/par.c): "<block>" lines open a block,
+// "key = value" lines set a key, "#" starts a comment.
+struct tmpg_input {
+    std::map<std::string, std::string> kv;  // "block/key" -> value
+};
+
+static std::string trim(const std::string& s)
+{
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace((unsigned char)s[a])) ++a;
+    while (b > a && std::isspace((unsigned char)s[b - 1])) --b;
+    return s.substr(a, b - a);
+}
+
+int tmpg_input_open(const char* path, tmpg_input** out)
+{
+    if (!path || !out) return TMPG_EINVAL;
+    std::ifstream in(path);
+    if (!in) { g_create_err = std::string("cannot open input file ") + path; return TMPG_EINVAL; }
+    tmpg_input* t = new (std::nothrow) tmpg_input();
+    if (!t) return TMPG_ENOMEM;
+    std::string line, block;
+    while (std::getline(in, line)) {
+        size_t h = line.find('#');
+        if (h != std::string::npos) line = line.substr(0, h);
+        line = trim(line);
+        if (line.empty()) continue;
+        if (line.front() == '<') {
+            size_t e = line.find('>');
+            block = trim(line.substr(1, e == std::string::npos ? std::string::npos : e - 1));
+            continue;
+        }
+        size_t eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        t->kv[block + "/" + trim(line.substr(0, eq))] = trim(line.substr(eq + 1));
+    }
+    *out = t;
+    return TMPG_OK;
+}
+
+int tmpg_input_close(tmpg_input* in)
+{
+    delete in;
+    return TMPG_OK;
+}
+
+int tmpg_input_get_double(const tmpg_input* in, const char* block, const char* key, double def, double* val)
+{
+    if (!in || !block || !key || !val) return TMPG_EINVAL;
+    auto it = in->kv.find(std::string(block) + "/" + key);
+    if (it == in->kv.end()) { *val = def; return TMPG_OK; }
+    char* end = nullptr;
+    double d = std::strtod(it->second.c_str(), &end);
+    if (end == it->second.c_str()) return TMPG_EINVAL;
+    *val = d;
+    return TMPG_OK;
+}
+
+int tmpg_input_get_int(const tmpg_input* in, const char* block, const char* key, int def, int* val)
+{
+    double d;
+    int rc = tmpg_input_get_double(in, block, key, (double)def, &d);
+    if (rc) return rc;
+    *val = (int)std::lround(d);
+    return TMPG_OK;
+}
+
+}  // extern "C"

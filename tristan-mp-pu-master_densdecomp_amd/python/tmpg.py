@@ -1,0 +1,235 @@
+"""Python binding of libtmpg.so (the C ABI of include/tmpg.h), via ctypes.
+
+The names follow the reference's per-step routines: advance_b_half
+(advance_Bhalfstep), move (move_particles), deposit (deposit_particles),
+filter (apply_filter1_opt), advance_e (advance_Efield), add_current,
+step (one main-loop iteration, code/tristanmainloop.F90:107-274).
+
+Grid arrays at this boundary use the reference layout: Fortran column-major
+(mx, my, mz) with ghosts, held here as numpy float32 arrays of shape
+(mz, my, mx) (same memory).  Particles are (7, n) float32 rows x, y, z, u, v,
+w, ch in the reference's absolute 1-based coordinates.
+
+There is no CPU fallback: a missing or unloadable libtmpg.so raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libtmpg.so")
+
+Q_GATHER2 = 1
+Q_BXY_Z = 2
+COMPAT_DEFAULT = 3
+
+EXPORTS = [
+    "tmpg_create", "tmpg_destroy", "tmpg_last_error", "tmpg_sync",
+    "tmpg_set_fields", "tmpg_get_fields", "tmpg_set_currents", "tmpg_get_currents",
+    "tmpg_set_particles", "tmpg_count_particles", "tmpg_get_particles",
+    "tmpg_advance_b_half", "tmpg_advance_e", "tmpg_reset_currents", "tmpg_move", "tmpg_deposit",
+    "tmpg_push_deposit", "tmpg_fold_currents", "tmpg_filter", "tmpg_add_current", "tmpg_step",
+    "tmpg_diag", "tmpg_charge_density", "tmpg_track_continuity", "tmpg_load_uniform",
+    "tmpg_last_timings", "tmpg_input_open", "tmpg_input_close", "tmpg_input_get_int",
+    "tmpg_input_get_double",
+]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("order", ctypes.c_int32),
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("ntimes", ctypes.c_int32),
+        ("compat", ctypes.c_int32),
+        ("track_ids", ctypes.c_int32),
+        ("c", ctypes.c_float),
+        ("corr", ctypes.c_float),
+        ("qm", ctypes.c_float * 2),
+        ("q", ctypes.c_float * 2),
+        ("mass", ctypes.c_float * 2),
+        ("capacity", ctypes.c_int64 * 2),
+        ("tile", ctypes.c_int32 * 3),
+        ("reserved", ctypes.c_int32 * 5),
+    ]
+
+
+class TmpgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libtmpg.so; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise TmpgError(f"{path} is not built (run the package build first)")
+        lib = ctypes.CDLL(path)
+        lib.tmpg_last_error.restype = ctypes.c_char_p
+        lib.tmpg_last_error.argtypes = [ctypes.c_void_p]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def extent(dim, order, nx, ny, nz):
+    ng = 5 if order == 1 else 7
+    ngz = 5 if dim == 2 else ng
+    return nx + ng, ny + ng, (nz + ngz if dim == 3 else 1), ng, ngz
+
+
+class Context:
+    """One simulation domain on the current GPU (tmpg_ctx)."""
+
+    def __init__(self, dim, order, nx, ny, nz=1, ntimes=4, compat=COMPAT_DEFAULT, c=0.45, corr=1.025,
+                 qm=(1.0, -1.0), q=(1.0, -1.0), mass=(1.0, 1.0), capacity=(0, 0), track_ids=False,
+                 tile=(0, 0, 0)):
+        self.lib = load_library()
+        cfg = Config()
+        cfg.dim, cfg.order, cfg.nx, cfg.ny, cfg.nz = dim, order, nx, ny, (nz if dim == 3 else 1)
+        cfg.ntimes, cfg.compat, cfg.track_ids = ntimes, compat, int(bool(track_ids))
+        cfg.c, cfg.corr = c, corr
+        for s in range(2):
+            cfg.qm[s], cfg.q[s], cfg.mass[s], cfg.capacity[s] = qm[s], q[s], mass[s], capacity[s]
+        for a in range(3):
+            cfg.tile[a] = tile[a]
+        self.cfg = cfg
+        self.dim, self.order = dim, order
+        self.nx, self.ny, self.nz = nx, ny, (nz if dim == 3 else 1)
+        self.mx, self.my, self.mz, self.ng, self.ngz = extent(dim, order, nx, ny, self.nz)
+        h = ctypes.c_void_p()
+        rc = self.lib.tmpg_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise TmpgError(f"tmpg_create: {rc}: {self.lib.tmpg_last_error(None).decode()}")
+        self.h = h
+
+    # -- helpers
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise TmpgError(f"{what}: {rc}: {self.lib.tmpg_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tmpg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid_shape(self):
+        return (self.mz, self.my, self.mx)
+
+    def _grid(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.shape == self.grid_shape(), (a.shape, self.grid_shape())
+        return a
+
+    # -- state
+    def set_fields(self, f6):
+        arr = [self._grid(f6[q]) for q in range(6)]
+        self._chk(self.lib.tmpg_set_fields(self.h, *(_p(a) for a in arr)), "set_fields")
+
+    def get_fields(self):
+        out = np.zeros((6,) + self.grid_shape(), dtype=np.float32)
+        self._chk(self.lib.tmpg_get_fields(self.h, *(_p(out[q]) for q in range(6))), "get_fields")
+        return out
+
+    def set_currents(self, j3):
+        arr = [self._grid(j3[q]) for q in range(3)]
+        self._chk(self.lib.tmpg_set_currents(self.h, *(_p(a) for a in arr)), "set_currents")
+
+    def get_currents(self):
+        out = np.zeros((3,) + self.grid_shape(), dtype=np.float32)
+        self._chk(self.lib.tmpg_get_currents(self.h, *(_p(out[q]) for q in range(3))), "get_currents")
+        return out
+
+    def set_particles(self, species, prt, ids=None):
+        prt = np.ascontiguousarray(prt, dtype=np.float32)
+        n = prt.shape[1]
+        rows = [np.ascontiguousarray(prt[r]) for r in range(7)]
+        idp = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        self._chk(self.lib.tmpg_set_particles(self.h, species, ctypes.c_int64(n), *(_p(r) for r in rows),
+                                              _p(idp)), "set_particles")
+
+    def count(self, species):
+        n = ctypes.c_int64()
+        self._chk(self.lib.tmpg_count_particles(self.h, species, ctypes.byref(n)), "count_particles")
+        return n.value
+
+    def get_particles(self, species, with_ids=False):
+        n = self.count(species)
+        out = np.zeros((7, n), dtype=np.float32)
+        ids = np.zeros(n, dtype=np.int32) if with_ids else None
+        got = ctypes.c_int64()
+        self._chk(self.lib.tmpg_get_particles(self.h, species, ctypes.c_int64(n), ctypes.byref(got),
+                                              *(_p(out[r]) for r in range(7)), _p(ids)), "get_particles")
+        return (out, ids) if with_ids else out
+
+    def load_uniform(self, species, ppc, drift_gb, theta, seed):
+        self._chk(self.lib.tmpg_load_uniform(self.h, species, ppc, ctypes.c_float(drift_gb),
+                                             ctypes.c_float(theta), ctypes.c_uint64(seed)), "load_uniform")
+
+    # -- per-step kernels
+    def advance_b_half(self):
+        self._chk(self.lib.tmpg_advance_b_half(self.h), "advance_b_half")
+
+    def advance_e(self):
+        self._chk(self.lib.tmpg_advance_e(self.h), "advance_e")
+
+    def reset_currents(self):
+        self._chk(self.lib.tmpg_reset_currents(self.h), "reset_currents")
+
+    def move(self):
+        self._chk(self.lib.tmpg_move(self.h), "move")
+
+    def deposit(self):
+        self._chk(self.lib.tmpg_deposit(self.h), "deposit")
+
+    def push_deposit(self):
+        self._chk(self.lib.tmpg_push_deposit(self.h), "push_deposit")
+
+    def fold_currents(self):
+        self._chk(self.lib.tmpg_fold_currents(self.h), "fold_currents")
+
+    def filter(self, npasses):
+        self._chk(self.lib.tmpg_filter(self.h, npasses), "filter")
+
+    def add_current(self):
+        self._chk(self.lib.tmpg_add_current(self.h), "add_current")
+
+    def step(self, nsteps=1):
+        self._chk(self.lib.tmpg_step(self.h, nsteps), "step")
+
+    def sync(self):
+        self._chk(self.lib.tmpg_sync(self.h), "sync")
+
+    # -- diagnostics
+    def diag(self):
+        out = np.zeros(8, dtype=np.float64)
+        self._chk(self.lib.tmpg_diag(self.h, _p(out)), "diag")
+        return out
+
+    def charge_density(self):
+        out = np.zeros((self.nz, self.ny, self.nx), dtype=np.float64)
+        self._chk(self.lib.tmpg_charge_density(self.h, _p(out)), "charge_density")
+        return out
+
+    def track_continuity(self, on=True):
+        self._chk(self.lib.tmpg_track_continuity(self.h, int(bool(on))), "track_continuity")
+
+    def last_timings(self):
+        out = np.zeros(5, dtype=np.float64)
+        self._chk(self.lib.tmpg_last_timings(self.h, _p(out)), "last_timings")
+        return out
