@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Throughput of the per-step PIC loop on one MI355X.
+
+Workload (default): 3D pair Weibel, 3rd-order shape, 512 x 256 x 256 periodic
+grid, 4 particles per cell per species (268,435,456 particles), counter-
+streaming beams +-x at gamma*beta = 1, thermal spread 0.1c (theta = 0.01),
+c = 0.45, c/wp = 6 cells, 4 filter passes.  Synthetic initial state from the
+library's loader; fields start at zero.
+
+A step is one full main-loop iteration (B half step, push+deposit, B half
+step, 4 filter passes, E step + current).  value = particles x steps / time.
+
+roofline: the push+deposit phase, priced at 56 B per particle (read+write of
+the packed cell, 3 offsets, 3 momenta) + 48 B per cell (6 field components
+read, 3 current components read-modify-write), per step, divided by the
+phase's average duration measured with HIP events on the library's stream.
+cpu_baseline: the reference's own Fortran routines (oracle/_ref, compiled
+from its sources) on one core over a bounded sample of the same workload
+(64 x 32 x 32 cells, same ppc and plasma), or the C++ oracle if that binary
+is absent.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "tristan-mp-pu-master_densdecomp_amd")
+sys.path.insert(0, os.path.join(PKG, "python"))
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # BASELINE config "3D pair Weibel precision study" at full size
+    "weibel3d_ord3_512x256x256_ppc4": dict(dim=3, order=3, nx=512, ny=256, nz=256, ppc=4, gb=1.0, theta=0.01,
+                                           c=0.45, c_omp=6.0, me=1.0, mi=1.0, ntimes=4),
+    # BASELINE config "3D electron-ion two-stream" (2nd order)
+    "twostream3d_ord2_192cube_ppc16": dict(dim=3, order=2, nx=192, ny=192, nz=192, ppc=16, gb=0.2, theta=1e-4,
+                                           c=0.45, c_omp=8.0, me=1.0, mi=16.0, ntimes=4),
+    # BASELINE config "2D pair Weibel" (1st order)
+    "weibel2d_ord1_256x256_ppc8": dict(dim=2, order=1, nx=256, ny=256, nz=1, ppc=8, gb=0.5, theta=2.5e-3,
+                                       c=0.45, c_omp=10.0, me=1.0, mi=1.0, ntimes=4),
+}
+
+
+def normalisation(w):
+    """code/particles.F90:206-235 for ppc0 = 2*ppc (both species)."""
+    import numpy as np
+    g0 = np.sqrt(1.0 + w["gb"] ** 2) if w["gb"] >= 0 else 1.0
+    ppc0 = 2.0 * w["ppc"]
+    omp = w["c"] / w["c_omp"]
+    qe = -(omp * omp * g0) / ((ppc0 * 0.5) * (1.0 + w["me"] / w["mi"]))
+    qi = -qe
+    me = w["me"] * abs(qi)
+    mi = w["mi"] * abs(qi)
+    return dict(qe=qe, qi=qi, me=me, mi=mi, qme=qe / me, qmi=qi / mi, g0=g0, ppc0=ppc0)
+
+
+def make_context(w):
+    import tmpg
+    nm = normalisation(w)
+    ncell = w["nx"] * w["ny"] * (w["nz"] if w["dim"] == 3 else 1)
+    cap = ncell * w["ppc"]
+    ctx = tmpg.Context(w["dim"], w["order"], w["nx"], w["ny"], w["nz"], ntimes=w["ntimes"], c=w["c"],
+                       qm=(nm["qmi"], nm["qme"]), q=(nm["qi"], nm["qe"]), mass=(nm["mi"], nm["me"]),
+                       capacity=(cap, cap))
+    # ions drift +x, electrons -x (two counter-streaming pair beams)
+    theta_e = w["theta"] * w["mi"] / w["me"] if w["mi"] != w["me"] else w["theta"]
+    ctx.load_uniform(0, w["ppc"], w["gb"], w["theta"] if w["mi"] == w["me"] else 0.0, 1)
+    ctx.load_uniform(1, w["ppc"], -w["gb"], theta_e if w["mi"] == w["me"] else w["theta"], 2)
+    return ctx, ncell
+
+
+def cpu_reference_baseline(w, budget_s=20.0):
+    """reference Fortran (oracle/_ref) on a 64x32x32 sample, one core."""
+    import numpy as np
+    dim, order = w["dim"], w["order"]
+    exe = os.path.join(ROOT, "oracle", "_ref", f"tristan_ref_{dim}d_dd{order}_f32")
+    sx, sy, sz = (64, 32, 32) if dim == 3 else (256, 256, 1)
+    nm = normalisation(w)
+    ng = 5 if order == 1 else 7
+    ngz = 5 if dim == 2 else ng
+    mx, my, mz = sx + ng, sy + ng, (sz + ngz if dim == 3 else 1)
+    h, hz = ng // 2, ngz // 2
+    rng = np.random.default_rng(0)
+    ncell = sx * sy * sz
+    n = ncell * w["ppc"]
+
+    def species(sign):
+        pos = np.empty((3, n))
+        pos[0] = h + 1 + rng.random(n) * sx
+        pos[1] = h + 1 + rng.random(n) * sy
+        pos[2] = (hz + 1 + rng.random(n) * sz) if dim == 3 else hz + 1.5
+        th = np.sqrt(w["theta"])
+        mom = rng.standard_normal((3, n)) * th
+        mom[0] += sign * w["gb"]
+        return np.concatenate([pos, mom, np.ones((1, n))]).astype(np.float32)
+
+    if not os.path.exists(exe):
+        return None
+    inp = f"""<node_configuration>
+sizex = 1
+sizey = 1
+<time>
+c = {w['c']}
+<grid>
+mx0 = {sx}
+my0 = {sy}
+mz0 = {sz}
+<algorithm>
+highorder = 0
+Corr = 1.025
+ntimes = {w['ntimes']}
+<boundaries>
+periodicx = 1
+periodicy = 1
+periodicz = 1
+<particles>
+maxptl0 = {4 * n + 100000}
+ppc0 = {nm['ppc0']}
+delgam = {w['theta']}
+me = {w['me']}
+mi = {w['mi']}
+gamma0 = {nm['g0']}
+c_omp = {w['c_omp']}
+<problem>
+distr_dim = 3
+external_fields = 0
+user_part_bcs = 0
+"""
+    with tempfile.TemporaryDirectory() as td:
+        open(os.path.join(td, "input"), "w").write(inp)
+        st = os.path.join(td, "state.bin")
+        with open(st, "wb") as fh:
+            fh.write(np.zeros(6 * mx * my * mz, np.float32).tobytes())
+            fh.write(np.array([n, n], np.int32).tobytes())
+            for s in (1, -1):
+                fh.write(species(s).tobytes())
+
+        def run(steps):
+            t0 = time.perf_counter()
+            subprocess.run([exe, "input", st, os.path.join(td, "out.bin"), str(steps), "0"], cwd=td, check=True,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+            return time.perf_counter() - t0
+
+        t0 = run(0)
+        t1 = run(1)
+        per_step = max(t1 - t0, 1e-6)
+        steps = max(1, min(20, int(budget_s / per_step)))
+        tk = run(steps)
+        per_step = (tk - t0) / steps
+    return dict(value=2 * n / per_step, unit="particle-steps/s", cores=1, kind="reference",
+                sample=f"reference Fortran -O2 (amdflang), {sx}x{sy}x{sz} cells, {w['ppc']} ppc/species "
+                       f"({2 * n} particles), {steps} full main-loop steps, 1 thread, startup subtracted")
+
+
+def read_traffic(workload):
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(p):
+        try:
+            return json.load(open(p)).get("bytes_per_step")
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="weibel3d_ord3_512x256x256_ppc4", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one periodic domain per GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[a.workload]
+    import tmpg
+    ctx, ncell = make_context(w)
+    npart = ctx.count(0) + ctx.count(1)
+    ctx.step(a.warmup)
+    ctx.sync()
+    ctx.last_timings()          # discard warm-up timings
+    t0 = time.perf_counter()
+    ctx.step(a.steps)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms = ctx.last_timings()
+    dt = t1 - t0
+    value = npart * a.steps / dt
+    push_ms = ms[1] / max(ms[3], 1)
+    alg_bytes = 56.0 * npart + 48.0 * ncell
+    achieved = alg_bytes / (push_ms * 1e-3) / 1e9
+    traffic = read_traffic(a.workload)
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_reference_baseline(w)
+        except Exception as e:  # the baseline is reported, never required
+            print(f"cpu baseline failed: {e}", file=sys.stderr)
+    out = {
+        "metric": "particle-steps/s (3D ord-3 full step)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * dt / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (library loader: drifting Maxwell-Juttner beams, uniform positions)",
+        "config": {"workload": a.workload, "grid": [w["nx"], w["ny"], w["nz"]], "order": w["order"],
+                   "particles": npart, "ppc_per_species": w["ppc"], "filter_passes": w["ntimes"],
+                   "phase_ms": {"fields": ms[0] / ms[3], "push_deposit": push_ms, "filter": ms[2] / ms[3]}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

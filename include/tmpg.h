@@ -139,8 +139,11 @@ int tmpg_track_continuity(tmpg_ctx* ctx, int on);
 int tmpg_load_uniform(tmpg_ctx* ctx, int species, int ppc, float drift_gb,
                       float theta, uint64_t seed);
 
-/* timing: milliseconds of the last tmpg_step per phase:
- * [0] fields, [1] push+deposit, [2] filter/denoise, [3] sort, [4] total */
+/* timing (HIP events on the context's stream): milliseconds summed over the
+ * tmpg_step iterations since the previous call, per phase:
+ * [0] field stencils (B half steps, E step + add current, J reset),
+ * [1] push+deposit (including its field staging and particle re-sort),
+ * [2] filter or denoiser, [3] number of steps summed, [4] total */
 int tmpg_last_timings(tmpg_ctx* ctx, double ms[5]);
 
 /* input file: the reference's <block> / key = value # comment format

@@ -19,6 +19,46 @@ namespace {
 
 __device__ __forceinline__ int wrapd(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// all live particles of a species: tile segments of a, then the overflow list
+struct SlotView {
+    PBuf a, oa;
+    const int32_t* cnt;
+    const int32_t* ocnt;
+    int32_t tcap;
+    int64_t slots, ocap;
+    __host__ __device__ int64_t total() const { return slots + ocap; }
+    // returns the buffer and index of slot s, or false when s is empty
+    __device__ __forceinline__ bool at(int64_t s, const PBuf*& b, int64_t& idx) const
+    {
+        if (s < slots) {
+            const int64_t t = s / tcap;
+            const int l = (int)(s - t * tcap);
+            if (l >= min(cnt[t], tcap)) return false;
+            b = &a;
+            idx = s;
+            return true;
+        }
+        const int64_t o = s - slots;
+        if (o >= min((int64_t)ocnt[0], ocap)) return false;
+        b = &oa;
+        idx = o;
+        return true;
+    }
+};
+
+SlotView slot_view(const Species& sp)
+{
+    SlotView v;
+    v.a = sp.a;
+    v.oa = sp.oa;
+    v.cnt = sp.cnt_a;
+    v.ocnt = sp.ocnt;
+    v.tcap = sp.tcap;
+    v.slots = sp.slots;
+    v.ocap = sp.ocap;
+    return v;
+}
+
 template <int BS>
 __device__ __forceinline__ double block_sum(double v, double* sh)
 {
@@ -45,42 +85,47 @@ __global__ void __launch_bounds__(256) k_sumsq(const float* __restrict__ a, cons
     if (threadIdx.x == 0) atomicAdd(out, 0.5 * s);
 }
 
-__global__ void __launch_bounds__(256) k_kinetic(const float* __restrict__ u, const float* __restrict__ v,
-                                                 const float* __restrict__ w, const float* __restrict__ ch,
-                                                 int64_t n, double mc2, double* out)
+__global__ void __launch_bounds__(256) k_kinetic(SlotView sv, double mc2, double* out)
 {
     __shared__ double sh[4];
     double s = 0.0;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        double uu = (double)u[t] * u[t] + (double)v[t] * v[t] + (double)w[t] * w[t];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < sv.total();
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const PBuf* b;
+        int64_t i;
+        if (!sv.at(t, b, i)) continue;
+        double uu = (double)b->u[i] * b->u[i] + (double)b->v[i] * b->v[i] + (double)b->w[i] * b->w[i];
         double gm1 = uu / (sqrt(1.0 + uu) + 1.0);
-        s += (ch ? (double)ch[t] : 1.0) * gm1;
+        s += (b->ch ? (double)b->ch[i] : 1.0) * gm1;
     }
     s = block_sum<256>(s, sh);
     if (threadIdx.x == 0) atomicAdd(out, mc2 * s);
 }
 
 template <int DIM, int ORD>
-__global__ void __launch_bounds__(256) k_rho(const int32_t* __restrict__ cell, const float* __restrict__ ox,
-                                             const float* __restrict__ oy, const float* __restrict__ oz,
-                                             const float* __restrict__ ch, int64_t n, PackInfo pk, int nx, int ny,
-                                             int nz, double q, double* __restrict__ rho)
+__global__ void __launch_bounds__(256) k_rho(SlotView sv, PackInfo pk, int nx, int ny, int nz, double q,
+                                             double* __restrict__ rho)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const PBuf* bp;
+    int64_t p;
+    if (s >= sv.total() || !sv.at(s, bp, p)) return;
+    const PBuf& B = *bp;
     int i, j, k;
-    unpack_cell(pk, cell[p], i, j, k);
+    unpack_cell(pk, B.cell[p], i, j, k);
     using P = Piece<ORD>;
     float wx[P::cnt], wy[P::cnt], wz[P::cnt];
-    const int mx = piece_of<ORD>(ox[p]), my = piece_of<ORD>(oy[p]);
-    P::w(ox[p] - (float)mx, wx);
-    P::w(oy[p] - (float)my, wy);
+    const float ox = B.ox[p], oy = B.oy[p];
+    const int mx = piece_of<ORD>(ox), my = piece_of<ORD>(oy);
+    P::w(ox - (float)mx, wx);
+    P::w(oy - (float)my, wy);
     int mz = 0;
     if (DIM == 3) {
-        mz = piece_of<ORD>(oz[p]);
-        P::w(oz[p] - (float)mz, wz);
+        const float oz = B.oz[p];
+        mz = piece_of<ORD>(oz);
+        P::w(oz - (float)mz, wz);
     }
-    const double qq = ch ? q * (double)ch[p] : q;
+    const double qq = B.ch ? q * (double)B.ch[p] : q;
     const int nzr = (DIM == 3) ? P::cnt : 1;
     for (int c = 0; c < nzr; ++c) {
         int kk = (DIM == 3) ? wrapd(k + mz + P::first + c, nz) : 0;
@@ -212,10 +257,10 @@ int diag_energies(tmpg_ctx* ctx, double out[4])
     for (int s = 0; s < 2; ++s) {
         Species& sp = ctx->sp[s];
         if (sp.n == 0) continue;
-        int pb = (int)std::min<int64_t>((sp.n + 255) / 256, 4096);
+        SlotView sv = slot_view(sp);
+        int pb = (int)std::min<int64_t>((sv.total() + 255) / 256, 4096);
         double mc2 = (double)ctx->cfg.mass[s] * (double)ctx->cfg.c * (double)ctx->cfg.c;
-        hipLaunchKernelGGL(k_kinetic, dim3(pb), dim3(256), 0, ctx->stream, sp.u, sp.v, sp.w,
-                           sp.has_ch ? sp.ch : nullptr, sp.n, mc2, d + 2 + s);
+        hipLaunchKernelGGL(k_kinetic, dim3(pb), dim3(256), 0, ctx->stream, sv, mc2, d + 2 + s);
     }
     TMPG_HIP_CHECK(ctx, hipGetLastError());
     TMPG_HIP_CHECK(ctx, hipMemcpyAsync(out, d, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -230,12 +275,11 @@ int diag_rho(tmpg_ctx* ctx, double* rho)
     for (int s = 0; s < 2; ++s) {
         Species& sp = ctx->sp[s];
         if (sp.n == 0) continue;
-        dim3 grid((unsigned)((sp.n + 255) / 256));
+        SlotView sv = slot_view(sp);
+        dim3 grid((unsigned)((sv.total() + 255) / 256));
         const double q = ctx->cfg.q[s];
-        const float* ch = sp.has_ch ? sp.ch : nullptr;
 #define TMPG_RHO(D, O)                                                                                       \
-    hipLaunchKernelGGL((k_rho<D, O>), grid, dim3(256), 0, ctx->stream, sp.cell, sp.ox, sp.oy, sp.oz, ch, sp.n, \
-                       ctx->pk, g.nx, g.ny, g.nz, q, rho)
+    hipLaunchKernelGGL((k_rho<D, O>), grid, dim3(256), 0, ctx->stream, sv, ctx->pk, g.nx, g.ny, g.nz, q, rho)
         switch (g.dim * 10 + ctx->cfg.order) {
         case 21: TMPG_RHO(2, 1); break;
         case 22: TMPG_RHO(2, 2); break;
@@ -301,25 +345,46 @@ int particles_load_uniform(tmpg_ctx* ctx, int s, int ppc, float drift, float the
     Species& sp = ctx->sp[s];
     const Grid& g = ctx->g;
     const int64_t count = g.n * (int64_t)ppc;
-    if (sp.n + count > sp.cap) {
-        ctx->err = "tmpg_load_uniform: capacity " + std::to_string(sp.cap) + " < " + std::to_string(sp.n + count);
-        return TMPG_EOVERFLOW;
+    // existing particles first, the new ones appended, then binned together
+    PBuf old;
+    int64_t n0 = 0;
+    int rc = particles_flatten(ctx, s, old, &n0);
+    if (rc) { free_pbuf(old); return rc; }
+    const bool ids = sp.has_id;
+    if (ids && n0 + count > INT32_MAX) { free_pbuf(old); ctx->err = "too many particles for int32 ids"; return TMPG_EINVAL; }
+    PBuf flat;
+    rc = alloc_pbuf(ctx, flat, std::max<int64_t>(n0 + count, 1), false, ids);
+    if (rc) { free_pbuf(old); return rc; }
+    const size_t fb = (size_t)n0 * sizeof(float), ib = (size_t)n0 * sizeof(int32_t);
+    if (n0 > 0) {
+        hipMemcpyAsync(flat.cell, old.cell, ib, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemcpyAsync(flat.ox, old.ox, fb, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemcpyAsync(flat.oy, old.oy, fb, hipMemcpyDeviceToDevice, ctx->stream);
+        if (g.dim == 3) hipMemcpyAsync(flat.oz, old.oz, fb, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemcpyAsync(flat.u, old.u, fb, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemcpyAsync(flat.v, old.v, fb, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemcpyAsync(flat.w, old.w, fb, hipMemcpyDeviceToDevice, ctx->stream);
+        if (ids) hipMemcpyAsync(flat.id, old.id, ib, hipMemcpyDeviceToDevice, ctx->stream);
     }
     const int ntab = 4096;
     std::vector<float> tab = mj_inverse_cdf(theta, ntab);
     float* dtab = nullptr;
-    TMPG_HIP_CHECK(ctx, hipMalloc(&dtab, ntab * sizeof(float)));
-    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(dtab, tab.data(), ntab * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    dim3 grid((unsigned)((count + 255) / 256));
-    hipLaunchKernelGGL(k_load, grid, dim3(256), 0, ctx->stream, sp.cell, sp.ox, sp.oy, sp.oz, sp.u, sp.v, sp.w,
-                       sp.id, sp.n, count, ppc, g.dim, g.nx, g.ny, ctx->pk, drift, dtab, ntab,
-                       seed * 2 + (uint64_t)s, (int32_t)sp.n);
-    hipError_t e = hipGetLastError();
-    hipStreamSynchronize(ctx->stream);
-    hipFree(dtab);
-    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return TMPG_EHIP; }
-    sp.n += count;
-    return TMPG_OK;
+    hipError_t e = hipMalloc(&dtab, ntab * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(dtab, tab.data(), ntab * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && count > 0) {
+        dim3 grid((unsigned)((count + 255) / 256));
+        hipLaunchKernelGGL(k_load, grid, dim3(256), 0, ctx->stream, flat.cell, flat.ox, flat.oy, flat.oz, flat.u,
+                           flat.v, flat.w, flat.id, n0, count, ppc, g.dim, g.nx, g.ny, ctx->pk, drift, dtab, ntab,
+                           seed * 2 + (uint64_t)s, (int32_t)n0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (dtab) hipFree(dtab);
+    free_pbuf(old);
+    if (e != hipSuccess) { free_pbuf(flat); ctx->err = hipGetErrorString(e); return TMPG_EHIP; }
+    rc = particles_bin(ctx, s, flat, n0 + count, false, ids);
+    free_pbuf(flat);
+    return rc;
 }
 
 }  // namespace tmpg

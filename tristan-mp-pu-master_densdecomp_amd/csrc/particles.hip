@@ -1,306 +1,714 @@
-// particles.hip -- gather + Boris push + Esirkepov deposit, one thread per
-// particle, reading fields from HBM/L2 and depositing with global atomics.
+// particles.hip -- the particle push: gather + Boris + Esirkepov deposit with
+// the per-step re-sort by cell tile fused in.
 //
 // Reference: move_particles / mover_{1,2,3}ord
 // (code/particles_movedeposit.F90:63-88, 356-1271) and deposit_particles /
 // densdecomp_{1,2,3}ord (code/particles_movedeposit.F90:1281-2051,
-// code/particles.F90:678-1358).  The deposit uses the stored old position and
-// the push's displacement (the reference recomputes the old position from the
-// new momentum, x0 = x - u c/gamma; the two agree in exact arithmetic).
-// This kernel is the general path: it serves every particle in the flat
-// layout and the overflow list of the tiled push.
-#include "pic_math.h"
+// code/particles.F90:678-1358); the cell sort of reorder_particles
+// (code/particles.F90:394-497) is replaced by binning into tiles every step.
+//
+// k_push_tiled: one workgroup per cell tile.  The tile's E,B with the halo of
+//   the shape order are staged in LDS, the tile's particles (both species)
+//   stream through with coalesced SoA loads, currents accumulate in LDS with
+//   LDS atomics and are added to HBM once per tile.  Each pushed particle is
+//   written straight into the segment of the tile it lands in (wave-level
+//   slot reservation for the ones that stay), so the next step finds the
+//   particles binned again without a separate sort pass.
+// k_push_flat: one thread per particle with HBM fields and global atomics;
+//   serves the overflow list and the split move/deposit calls.
+//
+// The deposit uses the stored old position and the push's displacement; the
+// reference recomputes the old position from the new momentum
+// (x0 = x - u c/gamma), which agrees in exact arithmetic.
+#include <algorithm>
+#include <vector>
+
+#include "pic_push.h"
 #include "tmpg_internal.h"
 
 namespace tmpg {
 namespace {
 
-struct PushArgs {
-    int nx, ny, nz;
-    PackInfo pk;
-    float c, cinv, qm, q;
-    int quirk2;
-    const float* f6[6];   // gather sources: primal E,B (3D) or staggered E,B (2D)
-    float* j3[3];
-    // particles
-    int64_t n;
-    int32_t* cell;
-    float *ox, *oy, *oz, *u, *v, *w;
-    const float* ch;
-    // split move / deposit scratch
-    int32_t* cell_old;
-    float *ox_old, *oy_old, *oz_old, *dx, *dy, *dz;
+struct SpecArgs {
+    float qm, q;
+    int32_t tcap;
+    int64_t ocap;
+    PBuf a, b, ob;
+    const int32_t* cnt_a;
+    int32_t* cnt_b;
+    int32_t* ocnt;
 };
 
-__device__ __forceinline__ int wrapn(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+struct TiledArgs {
+    int nx, ny, nz;
+    PackInfo pk;
+    Tiling tl;
+    int fo;                 // field region origin: tile origin - fo
+    int FX, FY, FZ;         // staged field region
+    int JX, JY, JZ;         // LDS current region (origin: tile origin - 2)
+    float c, cinv;
+    int quirk2;
+    int remap;              // XCD-aware tile order
+    const float* f6[6];
+    float* j3[3];
+    SpecArgs sp[2];
+};
 
-template <int DIM, int ORD>
-__device__ __forceinline__ void gather(const PushArgs& a, int i, int j, int k, float ox, float oy, float oz,
-                                       float* E)
-{
-    float Spx[4], Sdx[4], Spy[4], Sdy[4], Spz[4], Sdz[4];
-    int dix, diy, diz;
-    const bool qk = a.quirk2 != 0;
-    gather_weights<ORD>(ox, qk, Spx, Sdx, dix);
-    gather_weights<ORD>(oy, qk, Spy, Sdy, diy);
-    // node range of nonzero weights: order 1 uses nodes 0..1 only
-    constexpr int N0 = (ORD == 1) ? 1 : 0, N1 = (ORD == 1) ? 3 : 4;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) E[q] = 0.0f;
-    if (DIM == 3) {
-        gather_weights<ORD>(oz, qk, Spz, Sdz, diz);
-        for (int c = N0; c < N1; ++c) {
-            int kk = wrapn(k - 1 + c, a.nz);
-            for (int b = N0; b < N1; ++b) {
-                int jj = wrapn(j - 1 + b, a.ny);
-                const int64_t row = (int64_t)a.nx * (jj + (int64_t)a.ny * kk);
-                float acc[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-                for (int t = N0; t < N1; ++t) {
-                    int ii = wrapn(i - 1 + t, a.nx);
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) acc[q] += a.f6[q][row + ii] * Spx[t];
-                }
-                float wyz = Spy[b] * Spz[c];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) E[q] += acc[q] * wyz;
-            }
-        }
-    } else {
-        // staggered arrays with matching primal/dual weights
-        // (code/particles_movedeposit.F90:1155-1170)
-        for (int b = N0; b < N1; ++b) {
-            int jp = wrapn(j - 1 + b, a.ny), jd = wrapn(j + diy - 1 + b, a.ny);
-            const int64_t rp = (int64_t)a.nx * jp, rd = (int64_t)a.nx * jd;
-#pragma unroll
-            for (int t = N0; t < N1; ++t) {
-                int ip = wrapn(i - 1 + t, a.nx), id = wrapn(i + dix - 1 + t, a.nx);
-                E[0] += a.f6[0][rp + id] * Sdx[t] * Spy[b];
-                E[1] += a.f6[1][rd + ip] * Spx[t] * Sdy[b];
-                E[2] += a.f6[2][rp + ip] * Spx[t] * Spy[b];
-                E[3] += a.f6[3][rd + ip] * Spx[t] * Sdy[b];
-                E[4] += a.f6[4][rp + id] * Sdx[t] * Spy[b];
-                E[5] += a.f6[5][rd + id] * Sdx[t] * Sdy[b];
-            }
-        }
-    }
-}
-
-// Esirkepov deposit of one particle: old cell (i,j,k), old offsets o*,
-// displacement d*, charge q.  cur holds -J*dt in the reference's units
-// (positive sign convention of code/particles.F90:789-811): the x current at
-// node a is q * sum_{a'<=a} dSx(a') * (Ay Az + dSy dSz / 12) with
-// A = S1 + dS/2, the reference's S1S1 + dS S1/2 + S1 dS/2 + dS dS/3.
-template <int DIM, int ORD>
-__device__ __forceinline__ void deposit(const PushArgs& a, int i, int j, int k, float ox, float oy, float oz,
-                                        float dx, float dy, float dz, float q)
-{
-    constexpr int W = Win<ORD>::W;
-    float S1x[W], dSx[W], S1y[W], dSy[W];
-    const int bx = axis_deposit<ORD>(ox, dx, S1x, dSx);
-    const int by = axis_deposit<ORD>(oy, dy, S1y, dSy);
-    float Ax[W], Ay[W], Cx[W], Cy[W];
-    float run_x = 0.0f, run_y = 0.0f;
-#pragma unroll
-    for (int t = 0; t < W; ++t) {
-        Ax[t] = S1x[t] + 0.5f * dSx[t];
-        Ay[t] = S1y[t] + 0.5f * dSy[t];
-        run_x += dSx[t];
-        run_y += dSy[t];
-        Cx[t] = q * run_x;
-        Cy[t] = q * run_y;
-    }
-    const float twelfth = 1.0f / 12.0f;
-    float* JXp = a.j3[0];
-    float* JYp = a.j3[1];
-    float* JZp = a.j3[2];
-    if (DIM == 3) {
-        float S1z[W], dSz[W], Az[W], Cz[W];
-        const int bz = axis_deposit<ORD>(oz, dz, S1z, dSz);
-        float run_z = 0.0f;
-#pragma unroll
-        for (int t = 0; t < W; ++t) {
-            Az[t] = S1z[t] + 0.5f * dSz[t];
-            run_z += dSz[t];
-            Cz[t] = q * run_z;
-        }
-        int ii[W], jj[W], kk[W];
-#pragma unroll
-        for (int t = 0; t < W; ++t) {
-            ii[t] = wrapn(i + bx + t, a.nx);
-            jj[t] = wrapn(j + by + t, a.ny);
-            kk[t] = wrapn(k + bz + t, a.nz);
-        }
-#pragma unroll
-        for (int c = 0; c < W; ++c) {
-#pragma unroll
-            for (int b = 0; b < W; ++b) {
-                const int64_t row = (int64_t)a.nx * (jj[b] + (int64_t)a.ny * kk[c]);
-                const float wyz = Ay[b] * Az[c] + twelfth * dSy[b] * dSz[c];
-                const float wxz = Az[c];  // Jy row factor pieces
-#pragma unroll
-                for (int t = 0; t < W; ++t) {
-                    // Jx: prefix along x, last node carries the telescoped zero
-                    if (t < W - 1) {
-                        float vx = Cx[t] * wyz;
-                        if (vx != 0.0f) atomicAdd(JXp + row + ii[t], vx);
-                    }
-                    // Jy: prefix along y
-                    if (b < W - 1) {
-                        float vy = Cy[b] * (Ax[t] * wxz + twelfth * dSx[t] * dSz[c]);
-                        if (vy != 0.0f) atomicAdd(JYp + row + ii[t], vy);
-                    }
-                    // Jz: prefix along z
-                    if (c < W - 1) {
-                        float vz = Cz[c] * (Ax[t] * Ay[b] + twelfth * dSx[t] * dSy[b]);
-                        if (vz != 0.0f) atomicAdd(JZp + row + ii[t], vz);
-                    }
-                }
-            }
-        }
-    } else {
-        // 2D: Jx, Jy by prefix sums, Jz directly from the z displacement
-        // (code/particles.F90:822-848)
-        const float qdz = -q * dz;
-        int ii[W], jj[W];
-#pragma unroll
-        for (int t = 0; t < W; ++t) {
-            ii[t] = wrapn(i + bx + t, a.nx);
-            jj[t] = wrapn(j + by + t, a.ny);
-        }
-#pragma unroll
-        for (int b = 0; b < W; ++b) {
-            const int64_t row = (int64_t)a.nx * jj[b];
-#pragma unroll
-            for (int t = 0; t < W; ++t) {
-                if (t < W - 1) {
-                    float vx = Cx[t] * Ay[b];
-                    if (vx != 0.0f) atomicAdd(JXp + row + ii[t], vx);
-                }
-                if (b < W - 1) {
-                    float vy = Cy[b] * Ax[t];
-                    if (vy != 0.0f) atomicAdd(JYp + row + ii[t], vy);
-                }
-                float vz = qdz * (Ax[t] * Ay[b] + twelfth * dSx[t] * dSy[b]);
-                if (vz != 0.0f) atomicAdd(JZp + row + ii[t], vz);
-            }
-        }
-    }
-}
-
-// MODE 0: fused push + deposit, 1: push only (keeps old position and
-// displacement), 2: deposit of the last push
-template <int DIM, int ORD, int MODE, bool HAS_CH>
-__global__ void __launch_bounds__(256) k_push(PushArgs a)
-{
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
+struct Part {
     int i, j, k;
-    float ox, oy, oz = 0.0f, dx, dy, dz;
-    if (MODE == 2) {
-        unpack_cell(a.pk, a.cell_old[p], i, j, k);
-        ox = a.ox_old[p];
-        oy = a.oy_old[p];
-        if (DIM == 3) oz = a.oz_old[p];
-        dx = a.dx[p];
-        dy = a.dy[p];
-        dz = a.dz[p];
-    } else {
-        const int32_t cl = a.cell[p];
-        unpack_cell(a.pk, cl, i, j, k);
-        ox = a.ox[p];
-        oy = a.oy[p];
-        if (DIM == 3) oz = a.oz[p];
-        float u = a.u[p], v = a.v[p], w = a.w[p];
-        float E[6];
-        gather<DIM, ORD>(a, i, j, k, ox, oy, oz, E);
-        const float se = 0.5f * a.qm, sb = 0.5f * a.qm * a.cinv;
-        boris(a.c, a.cinv, se * E[0], se * E[1], se * E[2], sb * E[3], sb * E[4], sb * E[5], u, v, w, dx, dy,
-              dz);
-        a.u[p] = u;
-        a.v[p] = v;
-        a.w[p] = w;
-        // new position
-        float nx_ = ox, ny_ = oy, nz_ = oz;
-        int ni = wrapn(i + advance_offset(nx_, dx), a.nx);
-        int nj = wrapn(j + advance_offset(ny_, dy), a.ny);
-        int nk = k;
-        if (DIM == 3) nk = wrapn(k + advance_offset(nz_, dz), a.nz);
-        a.cell[p] = pack_cell(a.pk, ni, nj, nk);
-        a.ox[p] = nx_;
-        a.oy[p] = ny_;
-        if (DIM == 3) a.oz[p] = nz_;
-        if (MODE == 1) {
-            a.cell_old[p] = cl;
-            a.ox_old[p] = ox;
-            a.oy_old[p] = oy;
-            if (DIM == 3) a.oz_old[p] = oz;
-            a.dx[p] = dx;
-            a.dy[p] = dy;
-            a.dz[p] = dz;
-            return;
+    float ox, oy, oz, u, v, w, ch;
+    int32_t id;
+};
+
+template <int DIM, bool HAS_CH, bool HAS_ID>
+__device__ __forceinline__ void load_part(const PBuf& b, int64_t s, const PackInfo& pk, Part& p)
+{
+    unpack_cell(pk, b.cell[s], p.i, p.j, p.k);
+    p.ox = b.ox[s];
+    p.oy = b.oy[s];
+    p.oz = (DIM == 3) ? b.oz[s] : 0.0f;
+    p.u = b.u[s];
+    p.v = b.v[s];
+    p.w = b.w[s];
+    p.ch = HAS_CH ? b.ch[s] : 1.0f;
+    p.id = HAS_ID ? b.id[s] : 0;
+}
+
+template <int DIM, bool HAS_CH, bool HAS_ID>
+__device__ __forceinline__ void store_part(const PBuf& b, int64_t s, const PackInfo& pk, const Part& p)
+{
+    b.cell[s] = pack_cell(pk, p.i, p.j, p.k);
+    b.ox[s] = p.ox;
+    b.oy[s] = p.oy;
+    if (DIM == 3) b.oz[s] = p.oz;
+    b.u[s] = p.u;
+    b.v[s] = p.v;
+    b.w[s] = p.w;
+    if (HAS_CH) b.ch[s] = p.ch;
+    if (HAS_ID) b.id[s] = p.id;
+}
+
+// write a pushed particle to slot `slot` of tile `t` in b, or to the
+// overflow list when the segment is full; returns the index used
+// (t*tcap+slot, or slots + list index, or -1 when lost)
+template <int DIM, bool HAS_CH, bool HAS_ID>
+__device__ __forceinline__ int64_t emit(const SpecArgs& s, const PackInfo& pk, int64_t slots, int64_t t,
+                                        int32_t slot, const Part& p)
+{
+    if (slot < s.tcap) {
+        const int64_t d = t * s.tcap + slot;
+        store_part<DIM, HAS_CH, HAS_ID>(s.b, d, pk, p);
+        return d;
+    }
+    const int oi = atomicAdd(s.ocnt + 1, 1);
+    if (oi < s.ocap) {
+        store_part<DIM, HAS_CH, HAS_ID>(s.ob, oi, pk, p);
+        return slots + oi;
+    }
+    atomicAdd(s.ocnt + 2, 1);
+    return -1;
+}
+
+// Boris push and position advance of p; returns the displacement
+template <int DIM, int ORD, class Src>
+__device__ __forceinline__ void push_one(const Src& src, int quirk2, float c, float cinv, float qm, int si, int sj,
+                                         int sk, Part& p, float& dx, float& dy, float& dz)
+{
+    float E[6];
+    gather<DIM, ORD>(src, quirk2 != 0, si, sj, sk, p.ox, p.oy, p.oz, E);
+    const float se = 0.5f * qm, sb = 0.5f * qm * cinv;
+    boris(c, cinv, se * E[0], se * E[1], se * E[2], sb * E[3], sb * E[4], sb * E[5], p.u, p.v, p.w, dx, dy, dz);
+}
+
+__device__ __forceinline__ void advance(Part& p, int dim, int nx, int ny, int nz, float dx, float dy, float dz)
+{
+    p.i = wrap1(p.i + advance_offset(p.ox, dx), nx);
+    p.j = wrap1(p.j + advance_offset(p.oy, dy), ny);
+    if (dim == 3) p.k = wrap1(p.k + advance_offset(p.oz, dz), nz);
+}
+
+// --------------------------------------------------------------- tiled
+template <int DIM, int ORD, bool HAS_CH, bool HAS_ID>
+__global__ void __launch_bounds__(512) k_push_tiled(TiledArgs a)
+{
+    extern __shared__ float sm[];
+    const int nb = gridDim.x;
+    int64_t t = blockIdx.x;
+    if (a.remap) t = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+    const Tiling& tl = a.tl;
+    const int ti = (int)(t % tl.ntx);
+    const int tj = (int)((t / tl.ntx) % tl.nty);
+    const int tk = (int)(t / ((int64_t)tl.ntx * tl.nty));
+    const int i0 = ti * tl.tx, j0 = tj * tl.ty, k0 = tk * tl.tz;
+    const int fplane = a.FX * a.FY * a.FZ;
+    const int jplane = a.JX * a.JY * a.JZ;
+    float* sf = sm;
+    double* sj = reinterpret_cast<double*>(sm + ((6 * fplane + 1) & ~1));
+    const int tid = threadIdx.x, NT = blockDim.x;
+    // stage E,B (primal in 3D) with the halo, wrapping periodically
+    for (int e = tid; e < fplane; e += NT) {
+        const int li = e % a.FX;
+        const int r = e / a.FX;
+        const int lj = r % a.FY;
+        const int lk = r / a.FY;
+        const int gi = wrap1(i0 - a.fo + li, a.nx);
+        const int gj = wrap1(j0 - a.fo + lj, a.ny);
+        const int gk = (DIM == 3) ? wrap1(k0 - a.fo + lk, a.nz) : 0;
+        const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sf[q * fplane + e] = a.f6[q][g];
+    }
+    for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
+    __syncthreads();
+
+    const LdsSrc src{sf, a.FX, a.FY, fplane};
+    const LdsSink snk{sj, a.JX, a.JY, jplane};
+    const int lane = __lane_id();
+    const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll 1
+    for (int s = 0; s < 2; ++s) {
+        const SpecArgs& sp = a.sp[s];
+        const int n = min(sp.cnt_a[t], sp.tcap);
+        const int64_t base = t * sp.tcap;
+        const int64_t slots = (int64_t)tl.ntiles * sp.tcap;
+        for (int p0 = 0; p0 < n; p0 += NT) {
+            const int pi = p0 + tid;
+            const bool active = pi < n;
+            Part p;
+            int64_t td = -1;
+            if (active) {
+                load_part<DIM, HAS_CH, HAS_ID>(sp.a, base + pi, a.pk, p);
+                const int li = p.i - i0, lj = p.j - j0, lk = p.k - k0;
+                const int oi = p.i, oj = p.j, ok = p.k;
+                const float ox = p.ox, oy = p.oy, oz = p.oz;
+                float dx, dy, dz;
+                push_one<DIM, ORD>(src, a.quirk2, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo,
+                                   (DIM == 3) ? lk + a.fo : 0, p, dx, dy, dz);
+                advance(p, DIM, a.nx, a.ny, a.nz, dx, dy, dz);
+                deposit<DIM, ORD>(snk, li + 2, lj + 2, (DIM == 3) ? lk + 2 : 0, ox, oy, oz, dx, dy, dz,
+                                  sp.q * p.ch);
+                (void)oi; (void)oj; (void)ok;
+                td = tile_of(tl, p.i, p.j, p.k);
+            }
+            // slots in b: one reservation per wave for the particles that stay
+            const bool stay = active && td == t;
+            const uint64_t m = __ballot(stay);
+            int32_t slot = -1;
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                int32_t b0 = 0;
+                if (lane == leader) b0 = atomicAdd(sp.cnt_b + t, __popcll(m));
+                b0 = __shfl(b0, leader, 64);
+                if (stay) slot = b0 + __popcll(m & ltmask);
+            }
+            if (active && !stay) slot = atomicAdd(sp.cnt_b + td, 1);
+            if (active) emit<DIM, HAS_CH, HAS_ID>(sp, a.pk, slots, td, slot, p);
         }
     }
-    const float q = HAS_CH ? a.q * a.ch[p] : a.q;
-    deposit<DIM, ORD>(a, i, j, k, ox, oy, oz, dx, dy, dz, q);
+    __syncthreads();
+    // one pass of global atomics for the tile's currents
+    for (int e = tid; e < jplane; e += NT) {
+        const int li = e % a.JX;
+        const int r = e / a.JX;
+        const int lj = r % a.JY;
+        const int lk = r / a.JY;
+        const int gi = wrap1(i0 - 2 + li, a.nx);
+        const int gj = wrap1(j0 - 2 + lj, a.ny);
+        const int gk = (DIM == 3) ? wrap1(k0 - 2 + lk, a.nz) : 0;
+        const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float v = (float)sj[q * jplane + e];
+            if (v != 0.0f) atomicAdd(a.j3[q] + g, v);
+        }
+    }
 }
 
-template <int DIM, int ORD, int MODE>
-void launch_push(const PushArgs& a, bool has_ch, hipStream_t st)
+// ---------------------------------------------------------------- flat
+struct FlatArgs {
+    int nx, ny, nz;
+    PackInfo pk;
+    Tiling tl;
+    float c, cinv;
+    int quirk2;
+    GlobalSrc src;
+    GlobalSink snk;
+    SpecArgs sp;
+    // input: a tile-segmented buffer (cnt_in != null) or a list (count at *lcount)
+    PBuf in;
+    const int32_t* cnt_in;
+    int32_t tcap_in;
+    const int32_t* lcount;
+    int64_t lcap;
+    int64_t scratch_off;     // index offset of the input in the scratch arrays
+    int64_t nthreads;
+    // scratch of the split move/deposit
+    int32_t* cell_old;
+    float *ox_old, *oy_old, *oz_old, *dxs, *dys, *dzs;
+};
+
+// MODE 0: push + deposit + re-bin, 1: push + re-bin keeping the old position
+// and displacement, 2: deposit of the last MODE-1 push
+template <int DIM, int ORD, int MODE, bool HAS_CH, bool HAS_ID>
+__global__ void __launch_bounds__(256) k_push_flat(FlatArgs a)
 {
-    dim3 grid((unsigned)((a.n + 255) / 256));
-    if (has_ch)
-        hipLaunchKernelGGL((k_push<DIM, ORD, MODE, true>), grid, dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_push<DIM, ORD, MODE, false>), grid, dim3(256), 0, st, a);
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nthreads) return;
+    if (a.cnt_in) {
+        const int64_t t = s / a.tcap_in;
+        const int l = (int)(s - t * a.tcap_in);
+        if (l >= min(a.cnt_in[t], a.tcap_in)) return;
+    } else {
+        const int64_t cnt = min((int64_t)*a.lcount, a.lcap);
+        if (s >= cnt) return;
+    }
+    Part p;
+    load_part<DIM, HAS_CH, HAS_ID>(a.in, s, a.pk, p);
+    const int64_t sc = a.scratch_off + s;
+    if (MODE == 2) {
+        int i, j, k;
+        unpack_cell(a.pk, a.cell_old[sc], i, j, k);
+        deposit<DIM, ORD>(a.snk, i, j, k, a.ox_old[sc], a.oy_old[sc], DIM == 3 ? a.oz_old[sc] : 0.0f, a.dxs[sc],
+                          a.dys[sc], a.dzs[sc], a.sp.q * p.ch);
+        return;
+    }
+    const int oi = p.i, oj = p.j, ok = p.k;
+    const float ox = p.ox, oy = p.oy, oz = p.oz;
+    float dx, dy, dz;
+    push_one<DIM, ORD>(a.src, a.quirk2, a.c, a.cinv, a.sp.qm, p.i, p.j, p.k, p, dx, dy, dz);
+    advance(p, DIM, a.nx, a.ny, a.nz, dx, dy, dz);
+    if (MODE == 0) deposit<DIM, ORD>(a.snk, oi, oj, ok, ox, oy, oz, dx, dy, dz, a.sp.q * p.ch);
+    const int64_t td = tile_of(a.tl, p.i, p.j, p.k);
+    const int32_t slot = atomicAdd(a.sp.cnt_b + td, 1);
+    const int64_t slots = (int64_t)a.tl.ntiles * a.sp.tcap;
+    const int64_t d = emit<DIM, HAS_CH, HAS_ID>(a.sp, a.pk, slots, td, slot, p);
+    if (MODE == 1 && d >= 0) {
+        a.cell_old[d] = pack_cell(a.pk, oi, oj, ok);
+        a.ox_old[d] = ox;
+        a.oy_old[d] = oy;
+        if (DIM == 3) a.oz_old[d] = oz;
+        a.dxs[d] = dx;
+        a.dys[d] = dy;
+        a.dzs[d] = dz;
+    }
 }
 
-template <int DIM, int ORD>
-void launch_mode(const PushArgs& a, int mode, bool has_ch, hipStream_t st)
+// ------------------------------------------------------------- binning
+__global__ void k_bin_count(const int32_t* __restrict__ cell, int64_t n, PackInfo pk, Tiling tl,
+                            int32_t* __restrict__ cnt)
 {
-    if (mode == 0) launch_push<DIM, ORD, 0>(a, has_ch, st);
-    else if (mode == 1) launch_push<DIM, ORD, 1>(a, has_ch, st);
-    else launch_push<DIM, ORD, 2>(a, has_ch, st);
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int i, j, k;
+    unpack_cell(pk, cell[s], i, j, k);
+    atomicAdd(cnt + tile_of(tl, i, j, k), 1);
+}
+
+template <int DIM, bool HAS_CH, bool HAS_ID>
+__global__ void k_bin_scatter(PBuf in, int64_t n, PackInfo pk, Tiling tl, PBuf out, int32_t tcap,
+                              int32_t* __restrict__ cnt)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    Part p;
+    load_part<DIM, HAS_CH, HAS_ID>(in, s, pk, p);
+    const int64_t t = tile_of(tl, p.i, p.j, p.k);
+    const int32_t slot = atomicAdd(cnt + t, 1);
+    store_part<DIM, HAS_CH, HAS_ID>(out, t * tcap + slot, pk, p);
+}
+
+// compaction: tile segments of `in` -> consecutive positions of `out`
+// starting at off[t]
+template <int DIM, bool HAS_CH, bool HAS_ID>
+__global__ void k_flatten(PBuf in, const int32_t* __restrict__ cnt, int32_t tcap, int64_t nslots,
+                          const int64_t* __restrict__ off, PackInfo pk, PBuf out)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int64_t t = s / tcap;
+    const int l = (int)(s - t * tcap);
+    if (l >= min(cnt[t], tcap)) return;
+    Part p;
+    load_part<DIM, HAS_CH, HAS_ID>(in, s, pk, p);
+    store_part<DIM, HAS_CH, HAS_ID>(out, off[t] + l, pk, p);
+}
+
+template <int DIM, bool HAS_CH, bool HAS_ID>
+__global__ void k_copy_list(PBuf in, int64_t n, int64_t dst0, PackInfo pk, PBuf out)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    Part p;
+    load_part<DIM, HAS_CH, HAS_ID>(in, s, pk, p);
+    store_part<DIM, HAS_CH, HAS_ID>(out, dst0 + s, pk, p);
+}
+
+// ------------------------------------------------------- dispatch helpers
+template <class F>
+void dispatch(int dim, int order, bool ch, bool id, F&& f)
+{
+    // instantiate every (dim, order, ch, id) once; f is a generic lambda
+    auto go = [&](auto D, auto O) {
+        if (ch) { if (id) f(D, O, std::true_type{}, std::true_type{}); else f(D, O, std::true_type{}, std::false_type{}); }
+        else { if (id) f(D, O, std::false_type{}, std::true_type{}); else f(D, O, std::false_type{}, std::false_type{}); }
+    };
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using O1 = std::integral_constant<int, 1>;
+    using O2 = std::integral_constant<int, 2>;
+    using O3 = std::integral_constant<int, 3>;
+    if (dim == 2) {
+        if (order == 1) go(I2{}, O1{}); else if (order == 2) go(I2{}, O2{}); else go(I2{}, O3{});
+    } else {
+        if (order == 1) go(I3{}, O1{}); else if (order == 2) go(I3{}, O2{}); else go(I3{}, O3{});
+    }
+}
+
+SpecArgs spec_args(tmpg_ctx* ctx, int s)
+{
+    Species& sp = ctx->sp[s];
+    SpecArgs r;
+    r.qm = ctx->cfg.qm[s];
+    r.q = ctx->cfg.q[s];
+    r.tcap = sp.tcap;
+    r.ocap = sp.ocap;
+    r.a = sp.a;
+    r.b = sp.b;
+    r.ob = sp.ob;
+    r.cnt_a = sp.cnt_a;
+    r.cnt_b = sp.cnt_b;
+    r.ocnt = sp.ocnt;
+    return r;
+}
+
+FlatArgs flat_args(tmpg_ctx* ctx, int s)
+{
+    const Grid& g = ctx->g;
+    FlatArgs a;
+    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
+    a.pk = ctx->pk;
+    a.tl = ctx->tl;
+    a.c = ctx->cfg.c;
+    a.cinv = 1.0f / ctx->cfg.c;
+    a.quirk2 = (ctx->cfg.compat & TMPG_Q_GATHER2) ? 1 : 0;
+    for (int q = 0; q < 6; ++q) a.src.f[q] = (g.dim == 3) ? ctx->prim[q] : ctx->f[EX + q];
+    a.src.nx = g.nx; a.src.ny = g.ny; a.src.nz = g.nz;
+    for (int q = 0; q < 3; ++q) a.snk.j[q] = ctx->f[JX + q];
+    a.snk.nx = g.nx; a.snk.ny = g.ny; a.snk.nz = g.nz;
+    a.sp = spec_args(ctx, s);
+    Species& sp = ctx->sp[s];
+    a.cell_old = sp.cell_old; a.ox_old = sp.ox_old; a.oy_old = sp.oy_old; a.oz_old = sp.oz_old;
+    a.dxs = sp.dx; a.dys = sp.dy; a.dzs = sp.dz;
+    a.cnt_in = nullptr; a.lcount = nullptr; a.tcap_in = 0; a.lcap = 0; a.scratch_off = 0; a.nthreads = 0;
+    return a;
+}
+
+template <int DIM, int ORD, bool C, bool I>
+void launch_flat_mode(const FlatArgs& a, int mode, hipStream_t st)
+{
+    if (a.nthreads <= 0) return;
+    dim3 grid((unsigned)((a.nthreads + 255) / 256));
+    if (mode == 0) hipLaunchKernelGGL((k_push_flat<DIM, ORD, 0, C, I>), grid, dim3(256), 0, st, a);
+    else if (mode == 1) hipLaunchKernelGGL((k_push_flat<DIM, ORD, 1, C, I>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_push_flat<DIM, ORD, 2, C, I>), grid, dim3(256), 0, st, a);
+}
+
+void swap_buffers(Species& sp)
+{
+    std::swap(sp.a, sp.b);
+    std::swap(sp.cnt_a, sp.cnt_b);
+    std::swap(sp.oa, sp.ob);
 }
 
 }  // namespace
 
+// ---------------------------------------------------------------- host
+int alloc_pbuf(tmpg_ctx* ctx, PBuf& b, int64_t n, bool has_ch, bool has_id)
+{
+    b = PBuf();
+    if (n <= 0) return TMPG_OK;
+    const size_t fb = (size_t)n * sizeof(float), ib = (size_t)n * sizeof(int32_t);
+    auto al = [&](void** p, size_t bytes) -> int {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) {
+            ctx->err = "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e);
+            *p = nullptr;
+            return TMPG_ENOMEM;
+        }
+        return TMPG_OK;
+    };
+    int rc = 0;
+    rc |= al((void**)&b.cell, ib);
+    rc |= al((void**)&b.ox, fb);
+    rc |= al((void**)&b.oy, fb);
+    if (ctx->g.dim == 3) rc |= al((void**)&b.oz, fb);
+    rc |= al((void**)&b.u, fb);
+    rc |= al((void**)&b.v, fb);
+    rc |= al((void**)&b.w, fb);
+    if (has_ch) rc |= al((void**)&b.ch, fb);
+    if (has_id) rc |= al((void**)&b.id, ib);
+    if (rc) { free_pbuf(b); return TMPG_ENOMEM; }
+    return TMPG_OK;
+}
+
+void free_pbuf(PBuf& b)
+{
+    void* ps[] = {b.cell, b.ox, b.oy, b.oz, b.u, b.v, b.w, b.ch, b.id};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    b = PBuf();
+}
+
+void free_species(Species& s)
+{
+    free_pbuf(s.a);
+    free_pbuf(s.b);
+    free_pbuf(s.oa);
+    free_pbuf(s.ob);
+    void* ps[] = {s.cnt_a, s.cnt_b, s.ocnt, s.cell_old, s.ox_old, s.oy_old, s.oz_old, s.dx, s.dy, s.dz};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    s = Species();
+}
+
+int particles_bin(tmpg_ctx* ctx, int s, const PBuf& flat, int64_t n, bool has_ch, bool has_id)
+{
+    Species& sp = ctx->sp[s];
+    const Tiling& tl = ctx->tl;
+    free_species(sp);
+    sp.has_ch = has_ch;
+    sp.has_id = has_id;
+    TMPG_HIP_CHECK(ctx, hipMalloc(&sp.cnt_a, tl.ntiles * sizeof(int32_t)));
+    TMPG_HIP_CHECK(ctx, hipMalloc(&sp.cnt_b, tl.ntiles * sizeof(int32_t)));
+    TMPG_HIP_CHECK(ctx, hipMalloc(&sp.ocnt, 4 * sizeof(int32_t)));
+    TMPG_HIP_CHECK(ctx, hipMemsetAsync(sp.cnt_a, 0, tl.ntiles * sizeof(int32_t), ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipMemsetAsync(sp.ocnt, 0, 4 * sizeof(int32_t), ctx->stream));
+    if (n > 0)
+        hipLaunchKernelGGL(k_bin_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, flat.cell, n,
+                           ctx->pk, tl, sp.cnt_a);
+    std::vector<int32_t> cnt((size_t)tl.ntiles);
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), sp.cnt_a, cnt.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t mx = cnt.empty() ? 0 : *std::max_element(cnt.begin(), cnt.end());
+    const int64_t cells_per_tile = (int64_t)tl.tx * tl.ty * tl.tz;
+    const int64_t mean = (n + tl.ntiles - 1) / std::max<int64_t>(tl.ntiles, 1);
+    // slack for density changes: 25% above the fullest tile, at least 32
+    // slots or one particle per cell more
+    int64_t tcap = std::max<int64_t>({mx + mx / 4 + 32, mean + mean / 2, mx + cells_per_tile});
+    if (n == 0) tcap = 64;
+    tcap = (tcap + 63) / 64 * 64;
+    if (tcap > (int64_t)1 << 30) { ctx->err = "tile capacity too large"; return TMPG_EINVAL; }
+    sp.tcap = (int32_t)tcap;
+    sp.slots = tl.ntiles * tcap;
+    sp.ocap = std::max<int64_t>(65536, n / 64);
+    int rc = 0;
+    rc |= alloc_pbuf(ctx, sp.a, sp.slots, has_ch, has_id);
+    rc |= alloc_pbuf(ctx, sp.b, sp.slots, has_ch, has_id);
+    rc |= alloc_pbuf(ctx, sp.oa, sp.ocap, has_ch, has_id);
+    rc |= alloc_pbuf(ctx, sp.ob, sp.ocap, has_ch, has_id);
+    if (rc) return TMPG_ENOMEM;
+    TMPG_HIP_CHECK(ctx, hipMemsetAsync(sp.cnt_a, 0, tl.ntiles * sizeof(int32_t), ctx->stream));
+    if (n > 0) {
+        dim3 grid((unsigned)((n + 255) / 256));
+        const int dim = ctx->g.dim;
+        dispatch(dim, 1, has_ch, has_id, [&](auto D, auto, auto C, auto I) {
+            hipLaunchKernelGGL((k_bin_scatter<decltype(D)::value, decltype(C)::value, decltype(I)::value>), grid,
+                               dim3(256), 0, ctx->stream, flat, n, ctx->pk, tl, sp.a, sp.tcap, sp.cnt_a);
+        });
+        TMPG_HIP_CHECK(ctx, hipGetLastError());
+    }
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    sp.n = n;
+    return TMPG_OK;
+}
+
+int particles_flatten(tmpg_ctx* ctx, int s, PBuf& flat, int64_t* n_out)
+{
+    Species& sp = ctx->sp[s];
+    const Tiling& tl = ctx->tl;
+    *n_out = 0;
+    if (!sp.cnt_a) return TMPG_OK;
+    std::vector<int32_t> cnt((size_t)tl.ntiles);
+    int32_t oc[4];
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), sp.cnt_a, cnt.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(oc, sp.ocnt, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (oc[2] > 0) {
+        ctx->err = "particles lost: " + std::to_string(oc[2]) + " overflowed the overflow list";
+        return TMPG_EOVERFLOW;
+    }
+    std::vector<int64_t> off((size_t)tl.ntiles);
+    int64_t tot = 0;
+    for (size_t t = 0; t < cnt.size(); ++t) {
+        off[t] = tot;
+        tot += std::min(cnt[t], sp.tcap);
+    }
+    const int64_t nov = std::min<int64_t>(oc[0], sp.ocap);
+    const int64_t n = tot + nov;
+    int rc = alloc_pbuf(ctx, flat, std::max<int64_t>(n, 1), sp.has_ch, sp.has_id);
+    if (rc) return rc;
+    int64_t* doff = nullptr;
+    TMPG_HIP_CHECK(ctx, hipMalloc(&doff, off.size() * sizeof(int64_t)));
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(doff, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                                       ctx->stream));
+    dispatch(ctx->g.dim, 1, sp.has_ch, sp.has_id, [&](auto D, auto, auto C, auto I) {
+        constexpr int Dv = decltype(D)::value;
+        constexpr bool Cv = decltype(C)::value, Iv = decltype(I)::value;
+        if (sp.slots > 0)
+            hipLaunchKernelGGL((k_flatten<Dv, Cv, Iv>), dim3((unsigned)((sp.slots + 255) / 256)), dim3(256), 0,
+                               ctx->stream, sp.a, sp.cnt_a, sp.tcap, sp.slots, doff, ctx->pk, flat);
+        if (nov > 0)
+            hipLaunchKernelGGL((k_copy_list<Dv, Cv, Iv>), dim3((unsigned)((nov + 255) / 256)), dim3(256), 0,
+                               ctx->stream, sp.oa, nov, tot, ctx->pk, flat);
+    });
+    hipError_t e = hipGetLastError();
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(doff);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return TMPG_EHIP; }
+    *n_out = n;
+    return TMPG_OK;
+}
+
+int particles_rebalance(tmpg_ctx* ctx, int s)
+{
+    Species& sp = ctx->sp[s];
+    if (!sp.cnt_a) return TMPG_OK;
+    PBuf flat;
+    int64_t n = 0;
+    int rc = particles_flatten(ctx, s, flat, &n);
+    if (rc) { free_pbuf(flat); return rc; }
+    rc = particles_bin(ctx, s, flat, n, sp.has_ch, sp.has_id);
+    free_pbuf(flat);
+    return rc;
+}
+
+int particles_check(tmpg_ctx* ctx)
+{
+    // called at sync points: lost particles are an error, a filling overflow
+    // list triggers a re-bin with larger tiles segments
+    for (int s = 0; s < 2; ++s) {
+        Species& sp = ctx->sp[s];
+        if (!sp.ocnt) continue;
+        int32_t oc[4];
+        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(oc, sp.ocnt, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        if (oc[2] > 0) {
+            ctx->err = "particles lost: " + std::to_string(oc[2]) + " overflowed the overflow list";
+            return TMPG_EOVERFLOW;
+        }
+        if ((int64_t)oc[0] > sp.ocap / 8) {
+            int rc = particles_rebalance(ctx, s);
+            if (rc) return rc;
+        }
+    }
+    return TMPG_OK;
+}
+
 int particles_push(tmpg_ctx* ctx, int mode)
 {
     const Grid& g = ctx->g;
+    const Tiling& tl = ctx->tl;
     if (g.dim == 3 && mode != 2) {
         int rc = fields_primal(ctx);
         if (rc) return rc;
     }
-    for (int s = 0; s < 2; ++s) {
-        Species& sp = ctx->sp[s];
-        if (sp.n == 0) continue;
-        PushArgs a;
+    bool any = false;
+    for (int s = 0; s < 2; ++s) any |= ctx->sp[s].n > 0;
+    if (!any) return TMPG_OK;
+    if (mode != 2) {
+        for (int s = 0; s < 2; ++s) {
+            Species& sp = ctx->sp[s];
+            if (!sp.cnt_b) continue;
+            TMPG_HIP_CHECK(ctx, hipMemsetAsync(sp.cnt_b, 0, tl.ntiles * sizeof(int32_t), ctx->stream));
+            TMPG_HIP_CHECK(ctx, hipMemsetAsync(sp.ocnt + 1, 0, sizeof(int32_t), ctx->stream));
+        }
+    }
+    if (mode == 0) {
+        // the two species share the tile's staged fields: one launch
+        TiledArgs a;
         a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
         a.pk = ctx->pk;
+        a.tl = tl;
+        a.fo = (g.dim == 3) ? 1 : 2;
+        a.FX = tl.tx + (g.dim == 3 ? 3 : 4);
+        a.FY = tl.ty + (g.dim == 3 ? 3 : 4);
+        a.FZ = (g.dim == 3) ? tl.tz + 3 : 1;
+        a.JX = tl.tx + 5;
+        a.JY = tl.ty + 5;
+        a.JZ = (g.dim == 3) ? tl.tz + 5 : 1;
         a.c = ctx->cfg.c;
         a.cinv = 1.0f / ctx->cfg.c;
-        a.qm = ctx->cfg.qm[s];
-        a.q = ctx->cfg.q[s];
         a.quirk2 = (ctx->cfg.compat & TMPG_Q_GATHER2) ? 1 : 0;
+        a.remap = (tl.ntiles % 8 == 0) ? 1 : 0;
         for (int q = 0; q < 6; ++q) a.f6[q] = (g.dim == 3) ? ctx->prim[q] : ctx->f[EX + q];
         for (int q = 0; q < 3; ++q) a.j3[q] = ctx->f[JX + q];
-        a.n = sp.n;
-        a.cell = sp.cell; a.ox = sp.ox; a.oy = sp.oy; a.oz = sp.oz;
-        a.u = sp.u; a.v = sp.v; a.w = sp.w; a.ch = sp.ch;
-        a.cell_old = sp.cell_old; a.ox_old = sp.ox_old; a.oy_old = sp.oy_old; a.oz_old = sp.oz_old;
-        a.dx = sp.dx; a.dy = sp.dy; a.dz = sp.dz;
-        const int key = g.dim * 10 + ctx->cfg.order;
-        switch (key) {
-        case 21: launch_mode<2, 1>(a, mode, sp.has_ch, ctx->stream); break;
-        case 22: launch_mode<2, 2>(a, mode, sp.has_ch, ctx->stream); break;
-        case 23: launch_mode<2, 3>(a, mode, sp.has_ch, ctx->stream); break;
-        case 31: launch_mode<3, 1>(a, mode, sp.has_ch, ctx->stream); break;
-        case 32: launch_mode<3, 2>(a, mode, sp.has_ch, ctx->stream); break;
-        case 33: launch_mode<3, 3>(a, mode, sp.has_ch, ctx->stream); break;
-        default: ctx->err = "unsupported dim/order"; return TMPG_EINVAL;
+        for (int s = 0; s < 2; ++s) a.sp[s] = spec_args(ctx, s);
+        // a species without particles still needs valid count arrays
+        static_assert(sizeof(TiledArgs) < 4096, "kernel argument block");
+        const size_t lds = sizeof(float) * ((6 * (size_t)a.FX * a.FY * a.FZ + 1) & ~(size_t)1) +
+                           sizeof(double) * 3 * (size_t)a.JX * a.JY * a.JZ;
+        bool ch = ctx->sp[0].has_ch || ctx->sp[1].has_ch;
+        bool id = ctx->sp[0].has_id || ctx->sp[1].has_id;
+        if (ctx->sp[0].has_ch != ctx->sp[1].has_ch || ctx->sp[0].has_id != ctx->sp[1].has_id) {
+            // both species must carry the same optional arrays in one launch
+            ctx->err = "species differ in weights/ids";
+            return TMPG_ESTATE;
         }
+        dispatch(g.dim, ctx->cfg.order, ch, id, [&](auto D, auto O, auto C, auto I) {
+            constexpr int Dv = decltype(D)::value, Ov = decltype(O)::value;
+            constexpr bool Cv = decltype(C)::value, Iv = decltype(I)::value;
+            auto k = k_push_tiled<Dv, Ov, Cv, Iv>;
+            (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(k, dim3((unsigned)tl.ntiles), dim3(512), lds, ctx->stream, a);
+        });
         TMPG_HIP_CHECK(ctx, hipGetLastError());
+        // overflow list of the previous step
+        for (int s = 0; s < 2; ++s) {
+            Species& sp = ctx->sp[s];
+            if (!sp.cnt_a) continue;
+            FlatArgs f = flat_args(ctx, s);
+            f.in = sp.oa;
+            f.lcount = sp.ocnt;
+            f.lcap = sp.ocap;
+            f.nthreads = sp.ocap;
+            dispatch(g.dim, ctx->cfg.order, sp.has_ch, sp.has_id, [&](auto D, auto O, auto C, auto I) {
+                launch_flat_mode<decltype(D)::value, decltype(O)::value, decltype(C)::value, decltype(I)::value>(
+                    f, 0, ctx->stream);
+            });
+            TMPG_HIP_CHECK(ctx, hipGetLastError());
+        }
+    } else {
+        for (int s = 0; s < 2; ++s) {
+            Species& sp = ctx->sp[s];
+            if (!sp.cnt_a) continue;
+            for (int part = 0; part < 2; ++part) {
+                FlatArgs f = flat_args(ctx, s);
+                if (part == 0) {
+                    f.in = sp.a; f.cnt_in = sp.cnt_a; f.tcap_in = sp.tcap; f.nthreads = sp.slots;
+                    f.scratch_off = 0;
+                } else {
+                    f.in = sp.oa; f.lcount = sp.ocnt; f.lcap = sp.ocap; f.nthreads = sp.ocap;
+                    f.scratch_off = sp.slots;
+                }
+                dispatch(g.dim, ctx->cfg.order, sp.has_ch, sp.has_id, [&](auto D, auto O, auto C, auto I) {
+                    launch_flat_mode<decltype(D)::value, decltype(O)::value, decltype(C)::value,
+                                     decltype(I)::value>(f, mode, ctx->stream);
+                });
+                TMPG_HIP_CHECK(ctx, hipGetLastError());
+            }
+        }
+    }
+    if (mode != 2) {
+        for (int s = 0; s < 2; ++s) {
+            Species& sp = ctx->sp[s];
+            if (!sp.cnt_a) continue;
+            swap_buffers(sp);
+            // the new overflow list's count moves to slot 0
+            TMPG_HIP_CHECK(ctx, hipMemcpyAsync(sp.ocnt, sp.ocnt + 1, sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                               ctx->stream));
+        }
     }
     return TMPG_OK;
 }

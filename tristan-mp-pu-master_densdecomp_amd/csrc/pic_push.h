@@ -1,0 +1,209 @@
+// pic_push.h -- gather and Esirkepov deposit of one particle, written against
+// a field source and a current sink so the same arithmetic serves the tiled
+// kernel (LDS-staged fields, LDS currents) and the flat kernel (HBM fields,
+// global atomics).
+//
+// Reference: mover_{1,2,3}ord gathers (code/particles_movedeposit.F90:
+// 1137-1170 and the order-1/2 equivalents) and densdecomp_{1,2,3}ord
+// (code/particles.F90:678-1358).
+#pragma once
+#include "pic_math.h"
+
+namespace tmpg {
+
+__device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// gather nodes with nonzero weights: order 1 uses nodes 0..1, others -1..2
+template <int ORD> struct GatherRange { static constexpr int N0 = (ORD == 1) ? 1 : 0, N1 = (ORD == 1) ? 3 : 4; };
+
+// Field sources.  (i,j,k) are the particle's cell in the source's own
+// coordinates; node offsets are -2..+3.
+struct GlobalSrc {
+    const float* f[6];
+    int nx, ny, nz;
+    __device__ __forceinline__ int64_t row(int j, int k) const
+    {
+        return (int64_t)nx * (wrap1(j, ny) + (int64_t)ny * wrap1(k, nz));
+    }
+    __device__ __forceinline__ int col(int i) const { return wrap1(i, nx); }
+    __device__ __forceinline__ float get(int q, int64_t r, int c) const { return f[q][r + c]; }
+};
+
+struct LdsSrc {
+    const float* s;     // 6 component planes of fx*fy*fz floats
+    int fx, fy, plane;
+    __device__ __forceinline__ int row(int j, int k) const { return (j + fy * k) * fx; }
+    __device__ __forceinline__ int col(int i) const { return i; }
+    __device__ __forceinline__ float get(int q, int r, int c) const { return s[q * plane + r + c]; }
+};
+
+struct GlobalSink {
+    float* j[3];
+    int nx, ny, nz;
+    __device__ __forceinline__ int64_t row(int jj, int kk) const
+    {
+        return (int64_t)nx * (wrap1(jj, ny) + (int64_t)ny * wrap1(kk, nz));
+    }
+    __device__ __forceinline__ int col(int i) const { return wrap1(i, nx); }
+    __device__ __forceinline__ void add(int q, int64_t r, int c, float v) const { atomicAdd(j[q] + r + c, v); }
+};
+
+// Tile currents in LDS are accumulated in FP64: on gfx950 ds_add_f64 runs
+// about 3x faster than ds_add_f32 (measured on the 3D order-3 push), and the
+// tile sum is rounded to FP32 once, at the flush.
+struct LdsSink {
+    double* s;          // 3 component planes of jx*jy*jz doubles
+    int jx, jy, plane;
+    __device__ __forceinline__ int row(int jj, int kk) const { return (jj + jy * kk) * jx; }
+    __device__ __forceinline__ int col(int i) const { return i; }
+    __device__ __forceinline__ void add(int q, int r, int c, float v) const
+    {
+        atomicAdd(s + q * plane + r + c, (double)v);
+    }
+};
+
+// E, B at the particle; 3D from primal-averaged fields with primal weights
+// (code/particles_movedeposit.F90:1137-1153), 2D from the staggered fields
+// with matching primal/dual weights (:1155-1170).
+template <int DIM, int ORD, class Src>
+__device__ __forceinline__ void gather(const Src& src, bool quirk2, int i, int j, int k, float ox, float oy,
+                                       float oz, float* E)
+{
+    float Spx[4], Sdx[4], Spy[4], Sdy[4];
+    int dix, diy;
+    gather_weights<ORD>(ox, quirk2, Spx, Sdx, dix);
+    gather_weights<ORD>(oy, quirk2, Spy, Sdy, diy);
+    constexpr int N0 = GatherRange<ORD>::N0, N1 = GatherRange<ORD>::N1;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) E[q] = 0.0f;
+    if (DIM == 3) {
+        float Spz[4], Sdz[4];
+        int diz;
+        gather_weights<ORD>(oz, quirk2, Spz, Sdz, diz);
+        int cx[4];
+#pragma unroll
+        for (int t = N0; t < N1; ++t) cx[t] = src.col(i - 1 + t);
+#pragma unroll
+        for (int c = N0; c < N1; ++c) {
+#pragma unroll
+            for (int b = N0; b < N1; ++b) {
+                const auto r = src.row(j - 1 + b, k - 1 + c);
+                float acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int t = N0; t < N1; ++t)
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) acc[q] += src.get(q, r, cx[t]) * Spx[t];
+                const float wyz = Spy[b] * Spz[c];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) E[q] += acc[q] * wyz;
+            }
+        }
+    } else {
+        int cp[4], cd[4];
+#pragma unroll
+        for (int t = N0; t < N1; ++t) {
+            cp[t] = src.col(i - 1 + t);
+            cd[t] = src.col(i + dix - 1 + t);
+        }
+#pragma unroll
+        for (int b = N0; b < N1; ++b) {
+            const auto rp = src.row(j - 1 + b, 0), rd = src.row(j + diy - 1 + b, 0);
+#pragma unroll
+            for (int t = N0; t < N1; ++t) {
+                E[0] += src.get(0, rp, cd[t]) * Sdx[t] * Spy[b];
+                E[1] += src.get(1, rd, cp[t]) * Spx[t] * Sdy[b];
+                E[2] += src.get(2, rp, cp[t]) * Spx[t] * Spy[b];
+                E[3] += src.get(3, rd, cp[t]) * Spx[t] * Sdy[b];
+                E[4] += src.get(4, rp, cd[t]) * Sdx[t] * Spy[b];
+                E[5] += src.get(5, rd, cd[t]) * Sdx[t] * Sdy[b];
+            }
+        }
+    }
+}
+
+// Esirkepov deposit of one particle from old cell (i,j,k) / offsets o*, with
+// displacement d*, charge q.  The stored current follows the reference's
+// sign (E += cur): x current at node a = q * sum_{a'<=a} dSx(a') *
+// (Ay Az + dSy dSz/12), A = S1 + dS/2, which is the reference's
+// S1 S1 + dS S1/2 + S1 dS/2 + dS dS/3.  The last node of each prefix sum is
+// the telescoped zero and is not written.
+template <int DIM, int ORD, class Sink>
+__device__ __forceinline__ void deposit(const Sink& snk, int i, int j, int k, float ox, float oy, float oz,
+                                        float dx, float dy, float dz, float q)
+{
+    constexpr int W = Win<ORD>::W;
+    float S1x[W], dSx[W], S1y[W], dSy[W];
+    const int bx = axis_deposit<ORD>(ox, dx, S1x, dSx);
+    const int by = axis_deposit<ORD>(oy, dy, S1y, dSy);
+    float Ax[W], Ay[W], Cx[W], Cy[W];
+    float run_x = 0.0f, run_y = 0.0f;
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+        Ax[t] = S1x[t] + 0.5f * dSx[t];
+        Ay[t] = S1y[t] + 0.5f * dSy[t];
+        run_x += dSx[t];
+        run_y += dSy[t];
+        Cx[t] = q * run_x;
+        Cy[t] = q * run_y;
+    }
+    const float twelfth = 1.0f / 12.0f;
+    int cc[W];
+#pragma unroll
+    for (int t = 0; t < W; ++t) cc[t] = snk.col(i + bx + t);
+    if (DIM == 3) {
+        float S1z[W], dSz[W], Az[W], Cz[W];
+        const int bz = axis_deposit<ORD>(oz, dz, S1z, dSz);
+        float run_z = 0.0f;
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            Az[t] = S1z[t] + 0.5f * dSz[t];
+            run_z += dSz[t];
+            Cz[t] = q * run_z;
+        }
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+#pragma unroll
+            for (int b = 0; b < W; ++b) {
+                const auto r = snk.row(j + by + b, k + bz + c);
+                const float wyz = Ay[b] * Az[c] + twelfth * dSy[b] * dSz[c];
+#pragma unroll
+                for (int t = 0; t < W; ++t) {
+                    if (t < W - 1) {
+                        const float v = Cx[t] * wyz;
+                        if (v != 0.0f) snk.add(0, r, cc[t], v);
+                    }
+                    if (b < W - 1) {
+                        const float v = Cy[b] * (Ax[t] * Az[c] + twelfth * dSx[t] * dSz[c]);
+                        if (v != 0.0f) snk.add(1, r, cc[t], v);
+                    }
+                    if (c < W - 1) {
+                        const float v = Cz[c] * (Ax[t] * Ay[b] + twelfth * dSx[t] * dSy[b]);
+                        if (v != 0.0f) snk.add(2, r, cc[t], v);
+                    }
+                }
+            }
+        }
+    } else {
+        // 2D: Jz directly from the z displacement (code/particles.F90:822-848)
+        const float qdz = -q * dz;
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            const auto r = snk.row(j + by + b, 0);
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                if (t < W - 1) {
+                    const float v = Cx[t] * Ay[b];
+                    if (v != 0.0f) snk.add(0, r, cc[t], v);
+                }
+                if (b < W - 1) {
+                    const float v = Cy[b] * Ax[t];
+                    if (v != 0.0f) snk.add(1, r, cc[t], v);
+                }
+                const float v = qdz * (Ax[t] * Ay[b] + twelfth * dSx[t] * dSy[b]);
+                if (v != 0.0f) snk.add(2, r, cc[t], v);
+            }
+        }
+    }
+}
+
+}  // namespace tmpg

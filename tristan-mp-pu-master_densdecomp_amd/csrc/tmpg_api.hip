@@ -48,36 +48,11 @@ void dfree(void* p)
     if (p) (void)hipFree(p);
 }
 
-void free_species(Species& s)
-{
-    void* ps[] = {s.cell, s.ox, s.oy, s.oz, s.u, s.v, s.w, s.ch, s.id, s.cell2, s.ox2, s.oy2, s.oz2, s.u2, s.v2,
-                  s.w2, s.ch2, s.id2, s.cnt, s.cnt2, s.cell_old, s.ox_old, s.oy_old, s.oz_old, s.dx, s.dy, s.dz};
-    for (void* p : ps) dfree(p);
-    s = Species();
-}
-
-int alloc_species(tmpg_ctx* ctx, Species& s, int64_t cap, bool has_ch, bool ids)
-{
-    const bool d3 = ctx->g.dim == 3;
-    int rc = 0;
-    s.cap = cap;
-    s.has_ch = has_ch;
-    rc |= dalloc(ctx, &s.cell, cap);
-    rc |= dalloc(ctx, &s.ox, cap);
-    rc |= dalloc(ctx, &s.oy, cap);
-    if (d3) rc |= dalloc(ctx, &s.oz, cap);
-    rc |= dalloc(ctx, &s.u, cap);
-    rc |= dalloc(ctx, &s.v, cap);
-    rc |= dalloc(ctx, &s.w, cap);
-    if (has_ch) rc |= dalloc(ctx, &s.ch, cap);
-    if (ids) rc |= dalloc(ctx, &s.id, cap);
-    return rc ? TMPG_ENOMEM : TMPG_OK;
-}
-
 int alloc_split_scratch(tmpg_ctx* ctx, Species& s)
 {
+    // indexed like the particle's slot in the tile buffers, then the overflow list
     if (s.dx) return TMPG_OK;
-    const int64_t cap = s.cap;
+    const int64_t cap = s.slots + s.ocap;
     int rc = 0;
     rc |= dalloc(ctx, &s.cell_old, cap);
     rc |= dalloc(ctx, &s.ox_old, cap);
@@ -190,7 +165,29 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
     ctx->mx = cfg->nx + ng;
     ctx->my = cfg->ny + ng;
     ctx->mz = cfg->dim == 3 ? nz + ngz : 1;
-    ctx->tl = Tiling{ctx->g.nx, ctx->g.ny, ctx->g.nz, 1, 1, 1, 1};
+    {
+        // cell tiles of the sorted push (default 8^3 in 3D, 16^2 in 2D)
+        int tx = cfg->tile[0] > 0 ? cfg->tile[0] : (cfg->dim == 3 ? 8 : 16);
+        int ty = cfg->tile[1] > 0 ? cfg->tile[1] : (cfg->dim == 3 ? 8 : 16);
+        int tz = cfg->dim == 3 ? (cfg->tile[2] > 0 ? cfg->tile[2] : 8) : 1;
+        tx = std::min(tx, cfg->nx);
+        ty = std::min(ty, cfg->ny);
+        tz = std::min(tz, nz);
+        const size_t lds = sizeof(float) * (6 * (size_t)(tx + 4) * (ty + 4) * (cfg->dim == 3 ? tz + 3 : 1) + 1) +
+                           sizeof(double) * 3 * (size_t)(tx + 5) * (ty + 5) * (cfg->dim == 3 ? tz + 5 : 1);
+        if (lds > 160 * 1024) {
+            g_create_err = "tile too large for the 160 KiB LDS of a workgroup";
+            delete ctx;
+            return TMPG_EINVAL;
+        }
+        Tiling tl;
+        tl.tx = tx; tl.ty = ty; tl.tz = tz;
+        tl.ntx = (cfg->nx + tx - 1) / tx;
+        tl.nty = (cfg->ny + ty - 1) / ty;
+        tl.ntz = (nz + tz - 1) / tz;
+        tl.ntiles = (int64_t)tl.ntx * tl.nty * tl.ntz;
+        ctx->tl = tl;
+    }
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
@@ -213,12 +210,11 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
     (void)hipMemsetAsync(ctx->flags, 0, 16 * sizeof(int32_t), ctx->stream);
     for (auto& ev : ctx->ev) (void)hipEventCreate(&ev);
     for (int s = 0; s < 2; ++s) {
-        if (cfg->capacity[s] > 0) {
-            if (alloc_species(ctx, ctx->sp[s], cfg->capacity[s], false, cfg->track_ids != 0)) {
-                g_create_err = ctx->err;
-                tmpg_destroy(ctx);
-                return TMPG_ENOMEM;
-            }
+        // empty, binned species (the push needs valid tile counts for both)
+        if (particles_bin(ctx, s, PBuf(), 0, false, cfg->track_ids != 0)) {
+            g_create_err = ctx->err;
+            tmpg_destroy(ctx);
+            return TMPG_ENOMEM;
         }
     }
     e = hipStreamSynchronize(ctx->stream);
@@ -245,6 +241,7 @@ int tmpg_destroy(tmpg_ctx* ctx)
     for (auto& s : ctx->sp) free_species(s);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : ctx->prof) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TMPG_OK;
@@ -254,7 +251,8 @@ int tmpg_sync(tmpg_ctx* ctx)
 {
     if (!valid_ctx(ctx)) return TMPG_EINVAL;
     TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    return TMPG_OK;
+    ctx->steps_since_check = 0;
+    return particles_check(ctx);
 }
 
 int tmpg_set_fields(tmpg_ctx* ctx, const float* ex, const float* ey, const float* ez, const float* bx,
@@ -295,19 +293,12 @@ int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, co
         ctx->err = "null particle array";
         return TMPG_EINVAL;
     }
-    Species& sp = ctx->sp[species];
+    if (n > INT32_MAX && ctx->cfg.track_ids) { ctx->err = "too many particles for int32 ids"; return TMPG_EINVAL; }
     bool need_ch = false;
     if (ch)
         for (int64_t p = 0; p < n; ++p)
             if (ch[p] != 1.0f) { need_ch = true; break; }
     const bool ids = ctx->cfg.track_ids != 0;
-    if (n > sp.cap || (need_ch && !sp.ch)) {
-        int64_t cap = std::max<int64_t>(n, sp.cap);
-        free_species(sp);
-        int rc = alloc_species(ctx, sp, cap, need_ch, ids);
-        if (rc) return rc;
-    }
-    sp.has_ch = need_ch;
     std::vector<int32_t> cell((size_t)n);
     std::vector<float> o[3];
     for (auto& a : o) a.resize((size_t)n);
@@ -332,38 +323,39 @@ int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, co
         if (g.dim == 2) c[2] = 0;
         cell[p] = pack_cell(ctx->pk, c[0], c[1], c[2]);
     }
-    auto up = [&](void* d, const void* s, size_t bytes) -> int {
+    PBuf flat;
+    int rc = alloc_pbuf(ctx, flat, std::max<int64_t>(n, 1), need_ch, ids);
+    if (rc) return rc;
+    auto up = [&](void* d, const void* src, size_t bytes) -> int {
         if (!bytes) return TMPG_OK;
-        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, ctx->stream));
+        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream));
         return TMPG_OK;
     };
     const size_t fb = (size_t)n * sizeof(float);
-    int rc = 0;
-    rc |= up(sp.cell, cell.data(), (size_t)n * sizeof(int32_t));
-    rc |= up(sp.ox, o[0].data(), fb);
-    rc |= up(sp.oy, o[1].data(), fb);
-    if (g.dim == 3) rc |= up(sp.oz, o[2].data(), fb);
-    rc |= up(sp.u, u, fb);
-    rc |= up(sp.v, v, fb);
-    rc |= up(sp.w, w, fb);
-    if (need_ch) rc |= up(sp.ch, ch, fb);
-    if (ids && sp.id) {
-        std::vector<int32_t> tmp;
+    rc |= up(flat.cell, cell.data(), (size_t)n * sizeof(int32_t));
+    rc |= up(flat.ox, o[0].data(), fb);
+    rc |= up(flat.oy, o[1].data(), fb);
+    if (g.dim == 3) rc |= up(flat.oz, o[2].data(), fb);
+    rc |= up(flat.u, u, fb);
+    rc |= up(flat.v, v, fb);
+    rc |= up(flat.w, w, fb);
+    if (need_ch) rc |= up(flat.ch, ch, fb);
+    std::vector<int32_t> tmp;
+    if (ids) {
         const int32_t* src = id;
         if (!src) {
             tmp.resize((size_t)n);
             for (int64_t p = 0; p < n; ++p) tmp[(size_t)p] = (int32_t)p;
             src = tmp.data();
         }
-        rc |= up(sp.id, src, (size_t)n * sizeof(int32_t));
-        TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        rc |= up(flat.id, src, (size_t)n * sizeof(int32_t));
     }
-    if (rc) return rc;
-    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    sp.n = n;
+    if (!rc) rc = particles_bin(ctx, species, flat, n, need_ch, ids);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_pbuf(flat);
     ctx->move_pending = false;
     ctx->have_rho_prev = false;
-    return TMPG_OK;
+    return rc;
 }
 
 int tmpg_count_particles(tmpg_ctx* ctx, int species, int64_t* n)
@@ -378,36 +370,38 @@ int tmpg_get_particles(tmpg_ctx* ctx, int species, int64_t n_capacity, int64_t* 
 {
     if (!valid_ctx(ctx) || species < 0 || species > 1) return TMPG_EINVAL;
     Species& sp = ctx->sp[species];
-    const int64_t m = std::min<int64_t>(n_capacity, sp.n);
+    PBuf flat;
+    int64_t tot = 0;
+    int rc = particles_flatten(ctx, species, flat, &tot);
+    if (rc) { free_pbuf(flat); return rc; }
+    const int64_t m = std::min<int64_t>(n_capacity, tot);
     if (n) *n = m;
-    if (m <= 0) return TMPG_OK;
     const Grid& g = ctx->g;
-    std::vector<int32_t> cell((size_t)m);
+    std::vector<int32_t> cell((size_t)std::max<int64_t>(m, 0));
     std::vector<float> o[3];
-    for (auto& a : o) a.resize((size_t)m);
-    const size_t fb = (size_t)m * sizeof(float);
-    auto down = [&](void* d, const void* s, size_t bytes) -> int {
-        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    for (auto& a : o) a.resize(cell.size());
+    const size_t fb = (size_t)std::max<int64_t>(m, 0) * sizeof(float);
+    auto down = [&](void* d, const void* src, size_t bytes) -> int {
+        if (!bytes) return TMPG_OK;
+        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
         return TMPG_OK;
     };
-    int rc = 0;
-    rc |= down(cell.data(), sp.cell, (size_t)m * sizeof(int32_t));
-    rc |= down(o[0].data(), sp.ox, fb);
-    rc |= down(o[1].data(), sp.oy, fb);
-    if (g.dim == 3) rc |= down(o[2].data(), sp.oz, fb);
-    if (u) rc |= down(u, sp.u, fb);
-    if (v) rc |= down(v, sp.v, fb);
-    if (w) rc |= down(w, sp.w, fb);
-    if (id && sp.id) rc |= down(id, sp.id, (size_t)m * sizeof(int32_t));
-    if (rc) return rc;
-    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if (ch) {
-        if (sp.has_ch) {
-            TMPG_HIP_CHECK(ctx, hipMemcpy(ch, sp.ch, fb, hipMemcpyDeviceToHost));
-        } else {
-            std::fill(ch, ch + m, 1.0f);
-        }
+    if (m > 0) {
+        rc |= down(cell.data(), flat.cell, (size_t)m * sizeof(int32_t));
+        rc |= down(o[0].data(), flat.ox, fb);
+        rc |= down(o[1].data(), flat.oy, fb);
+        if (g.dim == 3) rc |= down(o[2].data(), flat.oz, fb);
+        if (u) rc |= down(u, flat.u, fb);
+        if (v) rc |= down(v, flat.v, fb);
+        if (w) rc |= down(w, flat.w, fb);
+        if (id && sp.has_id) rc |= down(id, flat.id, (size_t)m * sizeof(int32_t));
+        if (ch && sp.has_ch) rc |= down(ch, flat.ch, fb);
     }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    free_pbuf(flat);
+    if (rc) return rc;
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return TMPG_EHIP; }
+    if (ch && !sp.has_ch) std::fill(ch, ch + std::max<int64_t>(m, 0), 1.0f);
     const double bxy = ctx->h + 1, bz = ctx->hz + 1;
     for (int64_t p = 0; p < m; ++p) {
         int i, j, k;
@@ -506,8 +500,22 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
 {
     if (!valid_ctx(ctx) || nsteps < 0) return TMPG_EINVAL;
     int rc = 0;
-    hipEvent_t* ev = ctx->ev;
     for (int s = 0; s < nsteps && rc == 0; ++s) {
+        if (++ctx->steps_since_check >= 32) {
+            // lost particles are an error; a filling overflow list re-bins
+            ctx->steps_since_check = 0;
+            rc = particles_check(ctx);
+            if (rc) break;
+        }
+        // per-phase HIP events on the context's stream, read by tmpg_last_timings
+        const size_t need = (size_t)(ctx->prof_steps + 1) * 6;
+        while (ctx->prof.size() < need) {
+            hipEvent_t e;
+            TMPG_HIP_CHECK(ctx, hipEventCreate(&e));
+            ctx->prof.push_back(e);
+        }
+        hipEvent_t* ev = &ctx->prof[(size_t)ctx->prof_steps * 6];
+        ctx->prof_steps++;
         // code/tristanmainloop.F90:117-242 with the ghost fills and the
         // current fold absorbed into periodic indexing, reset_currents moved
         // before the push and add_current fused into the E update
@@ -533,19 +541,22 @@ int tmpg_last_timings(tmpg_ctx* ctx, double ms[5])
 {
     if (!valid_ctx(ctx) || !ms) return TMPG_EINVAL;
     TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    float t[5] = {0, 0, 0, 0, 0};
-    (void)hipEventElapsedTime(&t[0], ctx->ev[0], ctx->ev[1]);
-    float t2 = 0, t3 = 0;
-    (void)hipEventElapsedTime(&t[1], ctx->ev[1], ctx->ev[2]);
-    (void)hipEventElapsedTime(&t2, ctx->ev[2], ctx->ev[3]);
-    (void)hipEventElapsedTime(&t[2], ctx->ev[3], ctx->ev[4]);
-    (void)hipEventElapsedTime(&t3, ctx->ev[4], ctx->ev[5]);
-    (void)hipEventElapsedTime(&t[4], ctx->ev[0], ctx->ev[5]);
-    ms[0] = (double)t[0] + t2 + t3;
-    ms[1] = t[1];
-    ms[2] = t[2];
-    ms[3] = 0.0;
-    ms[4] = t[4];
+    for (int q = 0; q < 5; ++q) ms[q] = 0.0;
+    for (int s = 0; s < ctx->prof_steps; ++s) {
+        hipEvent_t* ev = &ctx->prof[(size_t)s * 6];
+        float t[5] = {0, 0, 0, 0, 0};
+        (void)hipEventElapsedTime(&t[0], ev[0], ev[1]);
+        (void)hipEventElapsedTime(&t[1], ev[1], ev[2]);
+        (void)hipEventElapsedTime(&t[2], ev[2], ev[3]);
+        (void)hipEventElapsedTime(&t[3], ev[3], ev[4]);
+        (void)hipEventElapsedTime(&t[4], ev[4], ev[5]);
+        ms[0] += (double)t[0] + t[2] + t[4];
+        ms[1] += t[1];
+        ms[2] += t[3];
+        ms[4] += (double)t[0] + t[1] + t[2] + t[3] + t[4];
+    }
+    ms[3] = (double)ctx->prof_steps;
+    ctx->prof_steps = 0;
     return TMPG_OK;
 }
 
@@ -595,16 +606,6 @@ int tmpg_load_uniform(tmpg_ctx* ctx, int species, int ppc, float drift_gb, float
     if (!valid_ctx(ctx) || species < 0 || species > 1 || ppc < 0 || theta < 0.0f) return TMPG_EINVAL;
     Species& sp = ctx->sp[species];
     if (sp.has_ch) { ctx->err = "tmpg_load_uniform: species has per-particle weights"; return TMPG_ESTATE; }
-    const int64_t need = sp.n + ctx->g.n * (int64_t)ppc;
-    if (need > sp.cap) {
-        if (sp.n > 0) {
-            ctx->err = "tmpg_load_uniform: capacity too small (set tmpg_config.capacity)";
-            return TMPG_EOVERFLOW;
-        }
-        free_species(sp);
-        int rc = alloc_species(ctx, sp, need, false, ctx->cfg.track_ids != 0);
-        if (rc) return rc;
-    }
     ctx->have_rho_prev = false;
     return particles_load_uniform(ctx, species, ppc, drift_gb, theta, seed);
 }

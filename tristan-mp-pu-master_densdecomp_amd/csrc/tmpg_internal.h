@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/tmpg.h"
 
@@ -19,30 +20,35 @@ struct Grid {
     int64_t n;  // cells
 };
 
-// Particles of one species in HBM: SoA, position as a packed integer cell
+// One SoA particle buffer: position as a packed integer cell
 // (i | j << bx | k << (bx+by)) plus FP32 offsets in [0,1), momentum
-// u = gamma*beta.  Stored in tile order: tile t owns slots
-// [t*tcap, t*tcap + count[t]) of every array (see tiles below).
-struct Species {
-    int64_t n = 0;        // live particles
-    int64_t cap = 0;      // slots per buffer (ntiles * tcap)
-    bool has_ch = false;
+// u = gamma*beta, optional weight ch and id.
+struct PBuf {
     int32_t* cell = nullptr;
     float *ox = nullptr, *oy = nullptr, *oz = nullptr;
     float *u = nullptr, *v = nullptr, *w = nullptr;
     float* ch = nullptr;
     int32_t* id = nullptr;
-    // second buffer: the sorted push writes here, then the two swap
-    int32_t* cell2 = nullptr;
-    float *ox2 = nullptr, *oy2 = nullptr, *oz2 = nullptr;
-    float *u2 = nullptr, *v2 = nullptr, *w2 = nullptr;
-    float* ch2 = nullptr;
-    int32_t* id2 = nullptr;
-    // per-tile counts of the current buffer and of the buffer being written
-    int32_t* cnt = nullptr;
-    int32_t* cnt2 = nullptr;
-    int32_t tcap = 0;      // slots per tile
-    // move/deposit split: old positions and displacements of the last move
+};
+
+// Particles of one species in HBM, binned by cell tile: tile t owns slots
+// [t*tcap, t*tcap + cnt[t]) of buffer a.  The push reads a and writes b
+// (each particle into the segment of the tile it lands in), then the two
+// swap.  A particle that finds its tile's segment full goes to the overflow
+// list, which the next push serves with the general (flat) kernel.
+struct Species {
+    int64_t n = 0;          // live particles
+    bool has_ch = false, has_id = false;
+    int32_t tcap = 0;       // slots per tile
+    int64_t slots = 0;      // ntiles * tcap
+    PBuf a, b;
+    int32_t* cnt_a = nullptr;   // per tile
+    int32_t* cnt_b = nullptr;
+    int64_t ocap = 0;           // overflow list capacity
+    PBuf oa, ob;
+    int32_t* ocnt = nullptr;    // [0] entries of oa, [1] of ob, [2] lost particles
+    // move/deposit split: old cell, offsets and displacement of the last
+    // move, indexed by the particle's slot in b (overflow: slots + index)
     int32_t* cell_old = nullptr;
     float *ox_old = nullptr, *oy_old = nullptr, *oz_old = nullptr;
     float *dx = nullptr, *dy = nullptr, *dz = nullptr;
@@ -72,6 +78,11 @@ struct Tiling {
     int64_t ntiles;
 };
 
+__host__ __device__ inline int64_t tile_of(const Tiling& t, int i, int j, int k)
+{
+    return (int64_t)(i / t.tx) + (int64_t)t.ntx * ((j / t.ty) + (int64_t)t.nty * (k / t.tz));
+}
+
 }  // namespace tmpg
 
 struct tmpg_ctx {
@@ -93,8 +104,13 @@ struct tmpg_ctx {
     bool track_cont = false;
     bool have_rho_prev = false;
     int32_t* flags = nullptr;  // device flags: [0] overflow
+    int32_t* hostcnt = nullptr;   // pinned: overflow counters copied after each tmpg_step
+    hipEvent_t cnt_ev = nullptr;
+    bool cnt_pending = false;
+    int steps_since_check = 0;
     hipEvent_t ev[8] = {};
-    double last_ms[5] = {0, 0, 0, 0, 0};
+    std::vector<hipEvent_t> prof;  // 6 events per tmpg_step iteration since the last read
+    int prof_steps = 0;
     std::string err;
 };
 
@@ -117,7 +133,18 @@ int fields_primal(tmpg_ctx* ctx);
 int fields_zero_j(tmpg_ctx* ctx);
 // mode 0: fused push+deposit, 1: move only, 2: deposit of the last move
 int particles_push(tmpg_ctx* ctx, int mode);
+// binning: (re)build the tile segments of species s from flat arrays of n
+// particles (device pointers); allocates the buffers
+int particles_bin(tmpg_ctx* ctx, int s, const PBuf& flat, int64_t n, bool has_ch, bool has_id);
+// compact copy of all live particles of species s into flat (device), n out
+int particles_flatten(tmpg_ctx* ctx, int s, PBuf& flat, int64_t* n);
+int particles_rebalance(tmpg_ctx* ctx, int s);
+int alloc_pbuf(tmpg_ctx* ctx, PBuf& b, int64_t n, bool has_ch, bool has_id);
+void free_pbuf(PBuf& b);
+void free_species(Species& s);
 int particles_load_uniform(tmpg_ctx* ctx, int s, int ppc, float drift, float theta, uint64_t seed);
+// apply fn to every live particle slot: kernels over the tiled layout use this
+int particles_check(tmpg_ctx* ctx);
 int diag_energies(tmpg_ctx* ctx, double out[4]);
 int diag_rho(tmpg_ctx* ctx, double* rho_dev);
 int diag_continuity(tmpg_ctx* ctx, double* maxres, double* rhonorm);
