@@ -224,6 +224,368 @@ __global__ void __launch_bounds__(512) k_push_tiled(TiledArgs a)
     }
 }
 
+// --------------------------------------------------- tiled, per-cell sums
+// 3D push whose deposit is reduced per cell before touching LDS atomics.
+// Phase A pushes the tile's particles like k_push_tiled, but keeps each
+// particle's deposit record (old offsets, displacement, charge, old cell) in
+// LDS instead of depositing it, unless its y/z shape window leaves the
+// 4-node window [-1,2] (a y or z cell crossing), in which case it deposits at
+// once.  Phase B counting-sorts the records by cell.  Phase C gives every
+// cell to two threads (split in z), which sum the Esirkepov stencils of all
+// of the cell's particles in registers -- x window [-2,3] (6 nodes, x
+// crossings included), y/z windows [-1,2] -- and add the sums to the tile's
+// LDS currents once.  That replaces ~300 LDS atomics per particle by ~250 per
+// cell.
+constexpr int CELLS_T = 512;        // threads per workgroup
+constexpr int CELLS_R = 2816;       // records per pass (both species)
+
+struct CellsLds {
+    int fplane, jplane, nc;
+    size_t f_off, j_off, rec_off, rq_off, rc_off, idx_off, cnt_off, start_off, fill_off, bytes;
+};
+
+__host__ __device__ inline CellsLds cells_lds(int FX, int FY, int FZ, int JX, int JY, int JZ, int nc)
+{
+    CellsLds L;
+    L.fplane = FX * FY * FZ;
+    L.jplane = JX * JY * JZ;
+    L.nc = nc;
+    size_t o = 0;
+    L.j_off = o;   o += sizeof(double) * 3 * (size_t)L.jplane;
+    L.f_off = o;   o += sizeof(float) * 6 * (size_t)L.fplane;
+    L.rec_off = o; o += sizeof(float) * 6 * (size_t)CELLS_R;
+    L.rq_off = o;  o += sizeof(float) * (size_t)CELLS_R;
+    L.cnt_off = o; o += sizeof(int) * (size_t)nc;
+    L.start_off = o; o += sizeof(int) * (size_t)nc;
+    L.fill_off = o; o += sizeof(int) * (size_t)nc;
+    L.rc_off = o;  o += sizeof(uint16_t) * (size_t)CELLS_R;
+    L.idx_off = o; o += sizeof(uint16_t) * (size_t)CELLS_R;
+    o = (o + 15) & ~(size_t)15;
+    o += 16;       // record counter
+    L.bytes = o;
+    return L;
+}
+
+// place the W window values starting at node `base` into an NW-node window
+// starting at node `first` (values outside are dropped: callers guarantee
+// they are zero)
+template <int W, int NW>
+__device__ __forceinline__ void place(const float* v, int base, int first, float* out)
+{
+#pragma unroll
+    for (int s = 0; s < NW; ++s) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int t = 0; t < W; ++t) acc = (s == base - first + t) ? v[t] : acc;
+        out[s] = acc;
+    }
+}
+
+// y/z shape window of a move inside nodes [-1,2]?
+template <int ORD>
+__device__ __forceinline__ bool fits4(float o, float d)
+{
+    const int m0 = piece_of<ORD>(o), m1 = piece_of<ORD>(o + d);
+    const int lo = (m0 < m1 ? m0 : m1) + Piece<ORD>::first;
+    const int hi = (m0 > m1 ? m0 : m1) + Piece<ORD>::first + Piece<ORD>::cnt - 1;
+    return lo >= -1 && hi <= 2;
+}
+
+template <int ORD, bool HAS_CH, bool HAS_ID>
+__global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
+{
+    extern __shared__ __align__(16) unsigned char smb[];
+    double* sj = reinterpret_cast<double*>(smb + L.j_off);
+    float* sf = reinterpret_cast<float*>(smb + L.f_off);
+    float* rec = reinterpret_cast<float*>(smb + L.rec_off);    // [6][R]: ox oy oz dx dy dz
+    float* rq = reinterpret_cast<float*>(smb + L.rq_off);
+    int* cnt = reinterpret_cast<int*>(smb + L.cnt_off);
+    int* start = reinterpret_cast<int*>(smb + L.start_off);
+    int* fill = reinterpret_cast<int*>(smb + L.fill_off);
+    uint16_t* rc = reinterpret_cast<uint16_t*>(smb + L.rc_off);
+    uint16_t* idx = reinterpret_cast<uint16_t*>(smb + L.idx_off);
+    // record counts packed in one word: low 16 bits cell-summed records
+    // (stored from 0 up), high 16 bits y/z crossers (stored from the top down)
+    unsigned* nrec = reinterpret_cast<unsigned*>(smb + L.bytes - 16);
+
+    const int nb = gridDim.x;
+    int64_t t = blockIdx.x;
+    if (a.remap) t = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+    const Tiling& tl = a.tl;
+    const int ti = (int)(t % tl.ntx);
+    const int tj = (int)((t / tl.ntx) % tl.nty);
+    const int tk = (int)(t / ((int64_t)tl.ntx * tl.nty));
+    const int i0 = ti * tl.tx, j0 = tj * tl.ty, k0 = tk * tl.tz;
+    const int fplane = L.fplane, jplane = L.jplane, nc = L.nc;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    for (int e = tid; e < fplane; e += NT) {
+        const int li = e % a.FX;
+        const int r = e / a.FX;
+        const int lj = r % a.FY;
+        const int lk = r / a.FY;
+        const int gi = wrap1(i0 - a.fo + li, a.nx);
+        const int gj = wrap1(j0 - a.fo + lj, a.ny);
+        const int gk = wrap1(k0 - a.fo + lk, a.nz);
+        const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sf[q * fplane + e] = a.f6[q][g];
+    }
+    for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
+    for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
+    if (tid == 0) *nrec = 0u;
+    __syncthreads();
+
+    const LdsSrc src{sf, a.FX, a.FY, fplane};
+    const LdsSink snk{sj, a.JX, a.JY, jplane};
+    const int lane = __lane_id();
+    const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    // ---- phase A: push, emit, record or deposit at once
+#pragma unroll 1
+    for (int s = 0; s < 2; ++s) {
+        const SpecArgs& sp = a.sp[s];
+        const int n = min(sp.cnt_a[t], sp.tcap);
+        const int64_t base = t * sp.tcap;
+        const int64_t slots = (int64_t)tl.ntiles * sp.tcap;
+        for (int p0 = 0; p0 < n; p0 += NT) {
+            const int pi = p0 + tid;
+            const bool active = pi < n;
+            Part p;
+            int64_t td = -1;
+            if (active) {
+                load_part<3, HAS_CH, HAS_ID>(sp.a, base + pi, a.pk, p);
+                const int li = p.i - i0, lj = p.j - j0, lk = p.k - k0;
+                const float ox = p.ox, oy = p.oy, oz = p.oz;
+                float dx, dy, dz;
+                push_one<3, ORD>(src, a.quirk2, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx, dy,
+                                 dz);
+                advance(p, 3, a.nx, a.ny, a.nz, dx, dy, dz);
+                const float q = sp.q * p.ch;
+                const bool fits = fits4<ORD>(oy, dy) && fits4<ORD>(oz, dz);
+                // cell-summed records grow from 0 up, y/z crossers from the
+                // top down; the two counts together never pass CELLS_R
+                int r = -1;
+                {
+                    const unsigned inc = fits ? 1u : 65536u;
+                    const unsigned old = atomicAdd(nrec, inc);
+                    const int lo = (int)(old & 0xffffu), hi = (int)(old >> 16);
+                    if (lo + hi < CELLS_R) r = fits ? lo : CELLS_R - 1 - hi;
+                    else atomicSub(nrec, inc);
+                }
+                const int c = li + tl.tx * (lj + tl.ty * lk);
+                if (r >= 0) {
+                    rec[0 * CELLS_R + r] = ox;
+                    rec[1 * CELLS_R + r] = oy;
+                    rec[2 * CELLS_R + r] = oz;
+                    rec[3 * CELLS_R + r] = dx;
+                    rec[4 * CELLS_R + r] = dy;
+                    rec[5 * CELLS_R + r] = dz;
+                    rq[r] = q;
+                    rc[r] = (uint16_t)c;
+                    if (fits) atomicAdd(cnt + c, 1);
+                } else {
+                    // record space exhausted: deposit this particle at once
+                    deposit<3, ORD>(snk, li + 2, lj + 2, lk + 2, ox, oy, oz, dx, dy, dz, q);
+                }
+                td = tile_of(tl, p.i, p.j, p.k);
+            }
+            const bool stay = active && td == t;
+            const uint64_t m = __ballot(stay);
+            int32_t slot = -1;
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                int32_t b0 = 0;
+                if (lane == leader) b0 = atomicAdd(sp.cnt_b + t, __popcll(m));
+                b0 = __shfl(b0, leader, 64);
+                if (stay) slot = b0 + __popcll(m & ltmask);
+            }
+            if (active && !stay) slot = atomicAdd(sp.cnt_b + td, 1);
+            if (active) emit<3, HAS_CH, HAS_ID>(sp, a.pk, slots, td, slot, p);
+        }
+    }
+    __syncthreads();
+
+    // ---- dense deposit of the y/z crossers (all lanes busy)
+    const int nr = (int)(*nrec & 0xffffu);
+    const int nd = (int)(*nrec >> 16);
+    for (int k = tid; k < nd; k += NT) {
+        const int r = CELLS_R - 1 - k;
+        const int c = rc[r];
+        const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
+        deposit<3, ORD>(snk, li + 2, lj + 2, lk + 2, rec[r], rec[CELLS_R + r], rec[2 * CELLS_R + r],
+                        rec[3 * CELLS_R + r], rec[4 * CELLS_R + r], rec[5 * CELLS_R + r], rq[r]);
+    }
+
+    // ---- phase B: counting sort of the cell-summed records by cell
+    if (tid < 64) {
+        // exclusive scan of cnt[0..nc) by the first wave
+        int carry = 0;
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+            const int c = c0 + tid;
+            const int v = c < nc ? cnt[c] : 0;
+            int x = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(x, o, 64);
+                if (tid >= o) x += y;
+            }
+            if (c < nc) start[c] = carry + x - v;
+            carry += __shfl(x, 63, 64);
+        }
+    }
+    __syncthreads();
+    for (int r = tid; r < nr; r += NT) {
+        const int c = rc[r];
+        const int pos = atomicAdd(fill + c, 1);
+        idx[start[c] + pos] = (uint16_t)r;
+    }
+    __syncthreads();
+
+    // ---- phase C: per-cell register sums, two threads per cell (z halves)
+    constexpr int W = Win<ORD>::W;
+    const float twelfth = 1.0f / 12.0f;
+    for (int wk = tid; wk < 2 * nc; wk += NT) {
+        const int c = wk >> 1, half = wk & 1;
+        const int r0 = start[c], r1 = r0 + cnt[c];
+        if (r1 == r0) continue;
+        const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
+        // z nodes of this half: Jx, Jy use z window slots {0,1} or {2,3};
+        // Jz prefix slots {0,1} or {2}
+        const int zb = half ? 2 : 0;
+        float ax[5][4][2], ay[6][3][2], az[6][4][2];
+#pragma unroll
+        for (int u = 0; u < 5; ++u)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) { ax[u][b][0] = 0.0f; ax[u][b][1] = 0.0f; }
+#pragma unroll
+        for (int u = 0; u < 6; ++u)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) { ay[u][b][0] = 0.0f; ay[u][b][1] = 0.0f; }
+#pragma unroll
+        for (int u = 0; u < 6; ++u)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) { az[u][b][0] = 0.0f; az[u][b][1] = 0.0f; }
+        for (int rr = r0; rr < r1; ++rr) {
+            const int r = idx[rr];
+            const float ox = rec[r], oy = rec[CELLS_R + r], oz = rec[2 * CELLS_R + r];
+            const float dx = rec[3 * CELLS_R + r], dy = rec[4 * CELLS_R + r], dz = rec[5 * CELLS_R + r];
+            const float q = rq[r];
+            float S1[W], dS[W];
+            float dSx[6], Ax[6], dSy[4], Ay[4], dSz[4], Az[4];
+            {
+                const int bx = axis_deposit<ORD>(ox, dx, S1, dS);
+                float S6[6];
+                place<W, 6>(S1, bx, -2, S6);
+                place<W, 6>(dS, bx, -2, dSx);
+#pragma unroll
+                for (int u = 0; u < 6; ++u) Ax[u] = S6[u] + 0.5f * dSx[u];
+            }
+            {
+                const int by = axis_deposit<ORD>(oy, dy, S1, dS);
+                float S4[4];
+                place<W, 4>(S1, by, -1, S4);
+                place<W, 4>(dS, by, -1, dSy);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) Ay[u] = S4[u] + 0.5f * dSy[u];
+            }
+            {
+                const int bz = axis_deposit<ORD>(oz, dz, S1, dS);
+                float S4[4];
+                place<W, 4>(S1, bz, -1, S4);
+                place<W, 4>(dS, bz, -1, dSz);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) Az[u] = S4[u] + 0.5f * dSz[u];
+            }
+            float Cx[5], Cy[3], Cz[3];
+            {
+                float run = 0.0f;
+#pragma unroll
+                for (int u = 0; u < 5; ++u) { run += dSx[u]; Cx[u] = q * run; }
+                run = 0.0f;
+#pragma unroll
+                for (int u = 0; u < 3; ++u) { run += dSy[u]; Cy[u] = q * run; }
+                run = 0.0f;
+#pragma unroll
+                for (int u = 0; u < 3; ++u) { run += dSz[u]; Cz[u] = q * run; }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int zc = zb + h;       // z window slot 0..3
+                const float azc = half ? Az[2 + h] : Az[h];
+                const float dzc = half ? dSz[2 + h] : dSz[h];
+                (void)zc;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const float wyz = Ay[b] * azc + twelfth * dSy[b] * dzc;
+#pragma unroll
+                    for (int u = 0; u < 5; ++u) ax[u][b][h] += Cx[u] * wyz;
+                }
+#pragma unroll
+                for (int u = 0; u < 6; ++u) {
+                    const float wxz = Ax[u] * azc + twelfth * dSx[u] * dzc;
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) ay[u][b][h] += Cy[b] * wxz;
+                }
+                // Jz prefix slot: half 0 -> {0,1}, half 1 -> {2} only
+                if (half == 0 || h == 0) {
+                    const float czc = half ? Cz[2] : Cz[h];
+#pragma unroll
+                    for (int u = 0; u < 6; ++u)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b)
+                            az[u][b][h] += czc * (Ax[u] * Ay[b] + twelfth * dSx[u] * dSy[b]);
+                }
+            }
+        }
+        // add the cell's sums to the tile currents: node (li-2+u, lj-1+b,
+        // lk-1+zslot) sits at LDS (li+u, lj+1+b, lk+1+zslot)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int zs = zb + h;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int row = snk.row(lj + 1 + b, lk + 1 + zs);
+#pragma unroll
+                for (int u = 0; u < 5; ++u)
+                    if (ax[u][b][h] != 0.0f) snk.add(0, row, li + u, ax[u][b][h]);
+            }
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const int row = snk.row(lj + 1 + b, lk + 1 + zs);
+#pragma unroll
+                for (int u = 0; u < 6; ++u)
+                    if (ay[u][b][h] != 0.0f) snk.add(1, row, li + u, ay[u][b][h]);
+            }
+            if (half == 0 || h == 0) {
+                const int zsz = half ? 2 : h;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int row = snk.row(lj + 1 + b, lk + 1 + zsz);
+#pragma unroll
+                    for (int u = 0; u < 6; ++u)
+                        if (az[u][b][h] != 0.0f) snk.add(2, row, li + u, az[u][b][h]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < jplane; e += NT) {
+        const int li = e % a.JX;
+        const int r = e / a.JX;
+        const int lj = r % a.JY;
+        const int lk = r / a.JY;
+        const int gi = wrap1(i0 - 2 + li, a.nx);
+        const int gj = wrap1(j0 - 2 + lj, a.ny);
+        const int gk = wrap1(k0 - 2 + lk, a.nz);
+        const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float v = (float)sj[q * jplane + e];
+            if (v != 0.0f) atomicAdd(a.j3[q] + g, v);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- flat
 struct FlatArgs {
     int nx, ny, nz;
@@ -656,12 +1018,22 @@ int particles_push(tmpg_ctx* ctx, int mode)
             ctx->err = "species differ in weights/ids";
             return TMPG_ESTATE;
         }
+        const int ncell_tile = tl.tx * tl.ty * tl.tz;
+        const CellsLds CL = cells_lds(a.FX, a.FY, a.FZ, a.JX, a.JY, a.JZ, ncell_tile);
+        const bool cells = g.dim == 3 && ctx->use_cell_sums && ncell_tile <= CELLS_T / 2 * 2 &&
+                           CL.bytes <= 160 * 1024;
         dispatch(g.dim, ctx->cfg.order, ch, id, [&](auto D, auto O, auto C, auto I) {
             constexpr int Dv = decltype(D)::value, Ov = decltype(O)::value;
             constexpr bool Cv = decltype(C)::value, Iv = decltype(I)::value;
-            auto k = k_push_tiled<Dv, Ov, Cv, Iv>;
-            (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipLaunchKernelGGL(k, dim3((unsigned)tl.ntiles), dim3(512), lds, ctx->stream, a);
+            if (Dv == 3 && cells) {
+                auto k = k_push_cells<Ov, Cv, Iv>;
+                (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)CL.bytes);
+                hipLaunchKernelGGL(k, dim3((unsigned)tl.ntiles), dim3(CELLS_T), CL.bytes, ctx->stream, a, CL);
+            } else {
+                auto k = k_push_tiled<Dv, Ov, Cv, Iv>;
+                (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                hipLaunchKernelGGL(k, dim3((unsigned)tl.ntiles), dim3(512), lds, ctx->stream, a);
+            }
         });
         TMPG_HIP_CHECK(ctx, hipGetLastError());
         // overflow list of the previous step

@@ -169,7 +169,7 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
         // cell tiles of the sorted push (default 8^3 in 3D, 16^2 in 2D)
         int tx = cfg->tile[0] > 0 ? cfg->tile[0] : (cfg->dim == 3 ? 8 : 16);
         int ty = cfg->tile[1] > 0 ? cfg->tile[1] : (cfg->dim == 3 ? 8 : 16);
-        int tz = cfg->dim == 3 ? (cfg->tile[2] > 0 ? cfg->tile[2] : 8) : 1;
+        int tz = cfg->dim == 3 ? (cfg->tile[2] > 0 ? cfg->tile[2] : 4) : 1;
         tx = std::min(tx, cfg->nx);
         ty = std::min(ty, cfg->ny);
         tz = std::min(tz, nz);

@@ -97,6 +97,7 @@ struct tmpg_ctx {
     float* tmp = nullptr;  // filter scratch (one component)
     tmpg::Species sp[2];
     bool move_pending = false;
+    bool use_cell_sums = true;  // 3D: per-cell register sums of the deposit
     // diagnostics (FP64)
     double* rho = nullptr;
     double* rho_prev = nullptr;
