@@ -33,16 +33,24 @@ sys.path.insert(0, os.path.join(PKG, "python"))
 
 HBM_PEAK_GBS = 8000.0
 
+# "load": (species, ppc, drift gamma*beta along x, theta) per loader call.
+# Weibel: like user/user_weibel.F90:252-292, two regions drifting -x and +x,
+# each holding both species.
 WORKLOADS = {
-    # BASELINE config "3D pair Weibel precision study" at full size
+    # 3D pair Weibel precision study, 3rd order, full size
     "weibel3d_ord3_512x256x256_ppc4": dict(dim=3, order=3, nx=512, ny=256, nz=256, ppc=4, gb=1.0, theta=0.01,
-                                           c=0.45, c_omp=6.0, me=1.0, mi=1.0, ntimes=4),
-    # BASELINE config "3D electron-ion two-stream" (2nd order)
-    "twostream3d_ord2_192cube_ppc16": dict(dim=3, order=2, nx=192, ny=192, nz=192, ppc=16, gb=0.2, theta=1e-4,
-                                           c=0.45, c_omp=8.0, me=1.0, mi=16.0, ntimes=4),
-    # BASELINE config "2D pair Weibel" (1st order)
+                                           c=0.45, c_omp=6.0, me=1.0, mi=1.0, ntimes=4,
+                                           load=[(0, 2, -1.0, 0.01), (1, 2, -1.0, 0.01),
+                                                 (0, 2, 1.0, 0.01), (1, 2, 1.0, 0.01)]),
+    # 3D electron-ion two-stream, 2nd order: electron beams +-0.2c over cold ions
+    "twostream3d_ord2_192cube_ppc16": dict(dim=3, order=2, nx=192, ny=192, nz=192, ppc=16, gb=0.2041, theta=1e-4,
+                                           c=0.45, c_omp=8.0, me=1.0, mi=16.0, ntimes=4,
+                                           load=[(0, 16, 0.0, 0.0), (1, 8, -0.2041, 1e-4), (1, 8, 0.2041, 1e-4)]),
+    # 2D pair Weibel, 1st order
     "weibel2d_ord1_256x256_ppc8": dict(dim=2, order=1, nx=256, ny=256, nz=1, ppc=8, gb=0.5, theta=2.5e-3,
-                                       c=0.45, c_omp=10.0, me=1.0, mi=1.0, ntimes=4),
+                                       c=0.45, c_omp=10.0, me=1.0, mi=1.0, ntimes=4,
+                                       load=[(0, 4, -0.5, 2.5e-3), (1, 4, -0.5, 2.5e-3),
+                                             (0, 4, 0.5, 2.5e-3), (1, 4, 0.5, 2.5e-3)]),
 }
 
 
@@ -67,10 +75,8 @@ def make_context(w):
     ctx = tmpg.Context(w["dim"], w["order"], w["nx"], w["ny"], w["nz"], ntimes=w["ntimes"], c=w["c"],
                        qm=(nm["qmi"], nm["qme"]), q=(nm["qi"], nm["qe"]), mass=(nm["mi"], nm["me"]),
                        capacity=(cap, cap))
-    # ions drift +x, electrons -x (two counter-streaming pair beams)
-    theta_e = w["theta"] * w["mi"] / w["me"] if w["mi"] != w["me"] else w["theta"]
-    ctx.load_uniform(0, w["ppc"], w["gb"], w["theta"] if w["mi"] == w["me"] else 0.0, 1)
-    ctx.load_uniform(1, w["ppc"], -w["gb"], theta_e if w["mi"] == w["me"] else w["theta"], 2)
+    for sp, ppc, gb, theta in w["load"]:
+        ctx.load_uniform(sp, ppc, gb, theta, 7 + 2 * sp + (gb > 0))
     return ctx, ncell
 
 
@@ -96,7 +102,8 @@ def cpu_reference_baseline(w, budget_s=20.0):
         pos[2] = (hz + 1 + rng.random(n) * sz) if dim == 3 else hz + 1.5
         th = np.sqrt(w["theta"])
         mom = rng.standard_normal((3, n)) * th
-        mom[0] += sign * w["gb"]
+        mom[0, : n // 2] -= w["gb"]          # half of each species per beam
+        mom[0, n // 2:] += w["gb"]
         return np.concatenate([pos, mom, np.ones((1, n))]).astype(np.float32)
 
     if not os.path.exists(exe):

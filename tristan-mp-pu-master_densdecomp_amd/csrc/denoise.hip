@@ -1,0 +1,898 @@
+// denoise.hip -- the learned current denoiser: a small periodic CNN whose
+// output is a vector potential A on the B grid; the current correction is
+// dJ = s * curl_b(A) with the backward-difference curl of the E update
+// (code/fields.F90:836-838), so div_b(dJ) = 0 and the Esirkepov charge
+// continuity of the deposit is kept to round-off.
+//
+// Network (DIM 2): 4 layers of 3x3 conv, width 32, GELU between layers;
+//         (DIM 3): 3 layers of 3x3x3 conv, width 16.
+// Input: the raw deposited current J / s (3 channels, padded to 4).
+//
+// Convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x16bf16_1k):
+// activations channels-last in BF16, FP32 accumulation.  The same kernel
+// computes the backward pass w.r.t. the input (flipped, transposed weights);
+// the weight gradient is its own MFMA kernel reducing over pixels.
+// Training (Adam) runs on the device; master weights are FP32.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tmpg_internal.h"
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace tmpg {
+namespace dn {
+
+__host__ __device__ inline uint16_t f2bf(float f)
+{
+    // round to nearest even
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b)
+{
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+struct Geo {
+    int dim, nx, ny, nz;
+    int64_t npix;       // pixels per image
+    int batch;
+};
+
+// periodic neighbour of pixel p (within its image) for tap t (3x3 or 3x3x3,
+// offsets -1..1, x fastest)
+__device__ __forceinline__ int64_t neighbour(const Geo& g, int64_t p, int tap)
+{
+    const int64_t img = p / g.npix;
+    int64_t r = p - img * g.npix;
+    int x = (int)(r % g.nx);
+    r /= g.nx;
+    int y = (int)(r % g.ny);
+    int z = (int)(r / g.ny);
+    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = (g.dim == 3) ? tap / 9 - 1 : 0;
+    x += dx; y += dy; z += dz;
+    x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+    y = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
+    if (g.dim == 3) z = z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z);
+    return img * g.npix + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+}
+
+// out[p][co] = bias[co] + sum_{tap,ci} W[co][tap*cin + ci] * in[nb(p,tap)][ci]
+// in: BF16 [P][cin] (cin multiple of 4); W: BF16 [cout_pad][kpad] with
+// kpad = taps*cin rounded up to 16 (zeros beyond); out: FP32 [P][cout]
+// (cout <= cout_pad, multiple of 16 for cout_pad).  One wave computes 16
+// pixels x cout_pad channels; 4 waves per block.
+// gelu_out (optional): BF16 gelu(out) [P][cout]; raw_out (optional): FP32.
+__device__ __forceinline__ float gelu(float z)
+{
+    const float k0 = 0.7978845608f, k1 = 0.044715f;
+    return 0.5f * z * (1.0f + tanhf(k0 * (z + k1 * z * z * z)));
+}
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    const float k0 = 0.7978845608f, k1 = 0.044715f;
+    const float u = k0 * (z + k1 * z * z * z);
+    const float th = tanhf(u);
+    return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * k0 * (1.0f + 3.0f * k1 * z * z);
+}
+
+template <int NT>   // NT = cout_pad / 16
+__global__ void __launch_bounds__(256) k_conv(Geo g, const uint16_t* __restrict__ in, int cin, int taps,
+                                              const uint16_t* __restrict__ W, int kpad,
+                                              const float* __restrict__ bias, int cout,
+                                              float* __restrict__ raw_out, uint16_t* __restrict__ gelu_out,
+                                              int gelu_stride)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t P = g.npix * g.batch;
+    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    if (p0 >= P) return;
+    const int i = lane & 15, kq = lane >> 4;
+    const int64_t p = p0 + i;
+    const bool pv = p < P;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int K = taps * cin;
+    for (int k0 = 0; k0 < kpad; k0 += 16) {
+        const int k = k0 + 4 * kq;
+        bf16x4 a = bf16x4{0, 0, 0, 0};
+        if (pv && k < K) {
+            const int tap = k / cin, ci = k - tap * cin;
+            const int64_t q = neighbour(g, p, tap);
+            const uint2 v = *reinterpret_cast<const uint2*>(in + q * cin + ci);
+            a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16);
+            a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
+        }
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int co = n * 16 + i;
+            const uint2 w = *reinterpret_cast<const uint2*>(W + (int64_t)co * kpad + k);
+            bf16x4 b;
+            b[0] = (short)(w.x & 0xffff); b[1] = (short)(w.x >> 16);
+            b[2] = (short)(w.y & 0xffff); b[3] = (short)(w.y >> 16);
+            acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+        }
+    }
+    // D[i][j]: j = lane % 16 (channel), i = 4*(lane/16) + r (pixel)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int co = n * 16 + (lane & 15);
+        if (co >= cout) continue;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t pp = p0 + 4 * kq + r;
+            if (pp >= P) continue;
+            const float z = acc[n][r] + bv;
+            if (raw_out) raw_out[pp * cout + co] = z;
+            if (gelu_out) gelu_out[pp * gelu_stride + co] = f2bf(gelu(z));
+        }
+    }
+}
+
+// dW[co][k] += sum_p dz[p][co] * patch(in)[p][k], db[co] += sum_p dz[p][co].
+// MFMA with M = co, N = k, reduction over 16 pixels per instruction; a block
+// sums a chunk of pixels for all output tiles and adds to dW atomically.
+__global__ void __launch_bounds__(256) k_conv_wgrad(Geo g, const uint16_t* __restrict__ dz, int dz_stride,
+                                                    int cout, int cout_pad, const uint16_t* __restrict__ in, int cin,
+                                                    int taps, int kpad, int64_t chunk, float* __restrict__ dW,
+                                                    float* __restrict__ db)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t P = g.npix * g.batch;
+    const int64_t pbeg = (int64_t)blockIdx.x * chunk;
+    const int64_t pend = pbeg + chunk < P ? pbeg + chunk : P;
+    const int nco = cout_pad / 16, nk = kpad / 16;
+    const int ntile = nco * nk;
+    const int K = taps * cin;
+    const int i = lane & 15, kq = lane >> 4;
+    // each wave owns tiles wave, wave+4, ...
+    for (int tile = wave; tile < ntile; tile += 4) {
+        const int ct = tile / nk, kt = tile % nk;
+        const int co = ct * 16 + i;           // A row (M)
+        const int kk = kt * 16 + i;           // B column (N)
+        int tap = 0, ci = 0;
+        const bool kv = kk < K;
+        if (kv) { tap = kk / cin; ci = kk - tap * cin; }
+        f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int64_t q0 = pbeg; q0 < pend; q0 += 16) {
+            bf16x4 a, b;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t p = q0 + 4 * kq + r;
+                short av = 0, bv = 0;
+                if (p < pend) {
+                    if (co < cout) av = (short)dz[p * dz_stride + co];
+                    if (kv) bv = (short)in[neighbour(g, p, tap) * cin + ci];
+                }
+                a[r] = av;
+                b[r] = bv;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        // D[m][n]: n = lane % 16 (k), m = 4*(lane/16) + r (co)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = ct * 16 + 4 * kq + r;
+            const int n = kt * 16 + (lane & 15);
+            if (m < cout && n < K) atomicAdd(dW + (int64_t)m * kpad + n, acc[r]);
+        }
+    }
+    // bias gradient: FP32 sum of the BF16 dz over the chunk
+    if (db) {
+        for (int c = threadIdx.x; c < cout; c += blockDim.x) {
+            float s = 0.0f;
+            for (int64_t p = pbeg; p < pend; ++p) s += bf2f(dz[p * dz_stride + c]);
+            atomicAdd(db + c, s);
+        }
+    }
+}
+
+// x (BF16, 4 channels) = J / s from the three FP32 current arrays of a batch
+__global__ void k_pack_input(const float* __restrict__ j, int64_t n, int batch, float inv_s,
+                             uint16_t* __restrict__ x)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * batch) return;
+    const int64_t b = t / n, p = t - b * n;
+    const float* jb = j + b * 3 * n;
+    x[t * 4 + 0] = f2bf(jb[p] * inv_s);
+    x[t * 4 + 1] = f2bf(jb[n + p] * inv_s);
+    x[t * 4 + 2] = f2bf(jb[2 * n + p] * inv_s);
+    x[t * 4 + 3] = 0;
+}
+
+// backward-difference curl on the periodic grid: c = curl_b(A), A [P][3]
+__global__ void k_curl(Geo g, const float* __restrict__ A, float* __restrict__ c)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.npix * g.batch) return;
+    // taps: x-1 -> 3 (dz=0,dy=0,dx=-1 -> tap 1+3+9?)  computed explicitly
+    const int64_t img = p / g.npix;
+    int64_t r = p - img * g.npix;
+    const int x = (int)(r % g.nx);
+    r /= g.nx;
+    const int y = (int)(r % g.ny);
+    const int z = (int)(r / g.ny);
+    auto at = [&](int xx, int yy, int zz) {
+        xx = xx < 0 ? xx + g.nx : xx;
+        yy = yy < 0 ? yy + g.ny : yy;
+        zz = zz < 0 ? zz + g.nz : zz;
+        return img * g.npix + xx + (int64_t)g.nx * (yy + (int64_t)g.ny * zz);
+    };
+    const int64_t px = at(x - 1, y, z), py = at(x, y - 1, z);
+    float cx = A[p * 3 + 2] - A[py * 3 + 2];
+    float cy = -(A[p * 3 + 2] - A[px * 3 + 2]);
+    float cz = (A[p * 3 + 1] - A[px * 3 + 1]) - (A[p * 3 + 0] - A[py * 3 + 0]);
+    if (g.dim == 3) {
+        const int64_t pz = at(x, y, z - 1);
+        cx -= A[p * 3 + 1] - A[pz * 3 + 1];
+        cy += A[p * 3 + 0] - A[pz * 3 + 0];
+    }
+    c[p * 3 + 0] = cx;
+    c[p * 3 + 1] = cy;
+    c[p * 3 + 2] = cz;
+}
+
+// adjoint of k_curl: dA = curl_b^T(gc)
+__global__ void k_curl_adj(Geo g, const float* __restrict__ gc, float* __restrict__ dA)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.npix * g.batch) return;
+    const int64_t img = p / g.npix;
+    int64_t r = p - img * g.npix;
+    const int x = (int)(r % g.nx);
+    r /= g.nx;
+    const int y = (int)(r % g.ny);
+    const int z = (int)(r / g.ny);
+    auto at = [&](int xx, int yy, int zz) {
+        xx = xx >= g.nx ? xx - g.nx : xx;
+        yy = yy >= g.ny ? yy - g.ny : yy;
+        zz = zz >= g.nz ? zz - g.nz : zz;
+        return img * g.npix + xx + (int64_t)g.nx * (yy + (int64_t)g.ny * zz);
+    };
+    const int64_t qx = at(x + 1, y, z), qy = at(x, y + 1, z);
+    // cx(p) = Az(p) - Az(p-y) [- Ay(p) + Ay(p-z)]; cy(p) = -Az(p) + Az(p-x) [+ Ax(p) - Ax(p-z)];
+    // cz(p) = Ay(p) - Ay(p-x) - Ax(p) + Ax(p-y)
+    float ax = -gc[p * 3 + 2] + gc[qy * 3 + 2];
+    float ay = gc[p * 3 + 2] - gc[qx * 3 + 2];
+    float az = gc[p * 3 + 0] - gc[qy * 3 + 0] - gc[p * 3 + 1] + gc[qx * 3 + 1];
+    if (g.dim == 3) {
+        const int64_t qz = at(x, y, z + 1);
+        ay += -gc[p * 3 + 0] + gc[qz * 3 + 0];
+        ax += gc[p * 3 + 1] - gc[qz * 3 + 1];
+    }
+    dA[p * 3 + 0] = ax;
+    dA[p * 3 + 1] = ay;
+    dA[p * 3 + 2] = az;
+}
+
+// pred = x + curl(A); loss += sum (pred - t)^2 / N; g = 2 (pred - t) / N.
+// x, t FP32 [batch][3][n] (the FP32 currents / s); curl [P][3]
+__global__ void k_loss(int64_t n, int batch, const float* __restrict__ jlo, const float* __restrict__ jt,
+                       float inv_s, const float* __restrict__ curl, float* __restrict__ grad, double* loss,
+                       double invN)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (t < n * batch) {
+        const int64_t b = t / n, p = t - b * n;
+        for (int c = 0; c < 3; ++c) {
+            const float pred = jlo[(b * 3 + c) * n + p] * inv_s + curl[t * 3 + c];
+            const float d = pred - jt[(b * 3 + c) * n + p] * inv_s;
+            s += (double)d * d;
+            if (grad) grad[t * 3 + c] = (float)(2.0 * invN) * d;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, s * invN);
+}
+
+// dz = dh * gelu'(z) in BF16 (dh FP32 [P][c], z FP32 [P][c])
+__global__ void k_gelu_back(int64_t m, const float* __restrict__ dh, const float* __restrict__ z,
+                            uint16_t* __restrict__ dz)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    dz[t] = f2bf(dh[t] * gelu_grad(z[t]));
+}
+
+__global__ void k_to_bf16(int64_t m, const float* __restrict__ a, int ca, uint16_t* __restrict__ b, int cb)
+{
+    // a [P][ca] -> b [P][cb] (cb >= ca, padded with zeros)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t P = m;
+    if (t >= P * cb) return;
+    const int64_t p = t / cb;
+    const int c = (int)(t - p * cb);
+    b[t] = c < ca ? f2bf(a[p * ca + c]) : (uint16_t)0;
+}
+
+// J += s * curl(A) on the library's three current arrays (batch 1)
+__global__ void k_apply(int64_t n, const float* __restrict__ curl, float s, float* jx, float* jy, float* jz)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    jx[p] += s * curl[p * 3 + 0];
+    jy[p] += s * curl[p * 3 + 1];
+    jz[p] += s * curl[p * 3 + 2];
+}
+
+__global__ void k_adam(int64_t m, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2,
+                       const uint8_t* __restrict__ mask)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    if (mask && !mask[t]) return;
+    const float gg = g[t];
+    const float a = b1 * m1[t] + (1.0f - b1) * gg;
+    const float b = b2 * m2[t] + (1.0f - b2) * gg * gg;
+    m1[t] = a;
+    m2[t] = b;
+    w[t] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// master FP32 weights [cout][kpad] -> BF16 forward weights [cout_pad][kpad]
+// and BF16 transposed/flipped weights for the input gradient:
+// WT[ci][tap'*cout_in + co] = W[co][tap*cin + ci] with tap' the mirrored tap
+__global__ void k_prep_weights(const float* __restrict__ w, int cout, int cout_pad, int cin, int taps, int kpad,
+                               uint16_t* __restrict__ wf, uint16_t* __restrict__ wt, int cinT, int cinT_pad,
+                               int kpadT)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (int64_t)cout_pad * kpad) {
+        const int co = (int)(t / kpad), k = (int)(t % kpad);
+        wf[t] = (co < cout && k < taps * cin) ? f2bf(w[(int64_t)co * kpad + k]) : (uint16_t)0;
+    }
+    // transposed: rows ci (cinT_pad of them), columns tap'*cinT + co  (cinT = cout rounded to 4)
+    if (wt && t < (int64_t)cinT_pad * kpadT) {
+        const int ci = (int)(t / kpadT), kk = (int)(t % kpadT);
+        uint16_t v = 0;
+        if (ci < cin && kk < taps * cinT) {
+            const int tapm = kk / cinT, co = kk - tapm * cinT;
+            const int tap = taps - 1 - tapm;   // mirrored offset
+            if (co < cout) v = f2bf(w[(int64_t)co * kpad + tap * cin + ci]);
+        }
+        wt[t] = v;
+    }
+}
+
+}  // namespace dn
+}  // namespace tmpg
+
+// ====================================================================== host
+using namespace tmpg;
+using namespace tmpg::dn;
+
+struct DLayer {
+    int cin, cout;          // real channel counts (cin includes the input pad to 4)
+    int cout_pad;           // multiple of 16 (MFMA N tiles)
+    int taps, kpad;         // K = taps*cin rounded to 16
+    int cinT, kpadT, cinT_pad;   // transposed conv for the input gradient
+    float* w = nullptr;     // FP32 master [cout][kpad]
+    float* b = nullptr;     // [cout]
+    float *gw = nullptr, *gb = nullptr, *m1w = nullptr, *m2w = nullptr, *m1b = nullptr, *m2b = nullptr;
+    uint16_t* wf = nullptr; // BF16 [cout_pad][kpad]
+    uint16_t* wt = nullptr; // BF16 [cinT_pad][kpadT]
+};
+
+struct tmpg_model {
+    int dim, width, nlayers;
+    std::vector<DLayer> L;
+    float scale = 0.0f;
+    int adam_t = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // work buffers, sized for the largest (pixels) seen
+    int64_t cap_pix = 0;
+    uint16_t* x0 = nullptr;              // input [P][4]
+    std::vector<float*> z;               // pre-activations [P][width]
+    std::vector<uint16_t*> h;            // activations [P][width]
+    float *A = nullptr, *curl = nullptr, *gcurl = nullptr, *dA = nullptr, *dh = nullptr;
+    uint16_t* dz = nullptr;
+    float *jlo = nullptr, *jt = nullptr;
+    double* dloss = nullptr;
+    // device-resident training set
+    std::vector<float*> set_lo, set_t;
+    int set_nx = 0, set_ny = 0, set_nz = 0;
+};
+
+namespace {
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+#define DN_CHECK(call)                                                                 \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            m->err = std::string(#call) + ": " + hipGetErrorString(e_);               \
+            return TMPG_EHIP;                                                          \
+        }                                                                              \
+    } while (0)
+
+template <class T>
+int dn_alloc(tmpg_model* m, T** p, int64_t n)
+{
+    hipError_t e = hipMalloc((void**)p, (size_t)std::max<int64_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) { m->err = std::string("hipMalloc: ") + hipGetErrorString(e); *p = nullptr; return TMPG_ENOMEM; }
+    return TMPG_OK;
+}
+
+void free_work(tmpg_model* m)
+{
+    auto f = [](void* p) { if (p) (void)hipFree(p); };
+    f(m->x0); f(m->A); f(m->curl); f(m->gcurl); f(m->dA); f(m->dh); f(m->dz); f(m->jlo); f(m->jt);
+    for (auto p : m->z) f(p);
+    for (auto p : m->h) f(p);
+    m->z.clear(); m->h.clear();
+    m->x0 = nullptr; m->A = m->curl = m->gcurl = m->dA = m->dh = nullptr; m->dz = nullptr;
+    m->jlo = m->jt = nullptr;
+    m->cap_pix = 0;
+}
+
+int ensure_work(tmpg_model* m, int64_t P)
+{
+    if (P <= m->cap_pix) return TMPG_OK;
+    free_work(m);
+    int rc = 0;
+    const int w = m->width;
+    rc |= dn_alloc(m, &m->x0, P * 4);
+    for (int l = 0; l < m->nlayers - 1; ++l) {
+        float* zz; uint16_t* hh;
+        rc |= dn_alloc(m, &zz, P * w);
+        rc |= dn_alloc(m, &hh, P * w);
+        m->z.push_back(zz); m->h.push_back(hh);
+    }
+    rc |= dn_alloc(m, &m->A, P * 3);
+    rc |= dn_alloc(m, &m->curl, P * 3);
+    rc |= dn_alloc(m, &m->gcurl, P * 3);
+    rc |= dn_alloc(m, &m->dA, P * 3);
+    rc |= dn_alloc(m, &m->dh, P * w);
+    rc |= dn_alloc(m, &m->dz, P * w);
+    rc |= dn_alloc(m, &m->jlo, P * 3);
+    rc |= dn_alloc(m, &m->jt, P * 3);
+    if (rc) return TMPG_ENOMEM;
+    m->cap_pix = P;
+    return TMPG_OK;
+}
+
+int prep_weights(tmpg_model* m)
+{
+    for (auto& l : m->L) {
+        const int64_t n1 = (int64_t)l.cout_pad * l.kpad, n2 = (int64_t)l.cinT_pad * l.kpadT;
+        hipLaunchKernelGGL(k_prep_weights, dim3((unsigned)cdiv(std::max(n1, n2), 256)), dim3(256), 0, m->stream,
+                           l.w, l.cout, l.cout_pad, l.cin, l.taps, l.kpad, l.wf, l.wt, l.cinT, l.cinT_pad, l.kpadT);
+    }
+    DN_CHECK(hipGetLastError());
+    return TMPG_OK;
+}
+
+void launch_conv(tmpg_model* m, const Geo& g, const uint16_t* in, int cin, int taps, const uint16_t* W, int kpad,
+                 int cout_pad, const float* bias, int cout, float* raw, uint16_t* gout, int gstride)
+{
+    const int64_t P = g.npix * g.batch;
+    dim3 grid((unsigned)cdiv(P, 64));
+    if (cout_pad == 16)
+        hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), 0, m->stream, g, in, cin, taps, W, kpad, bias, cout, raw,
+                           gout, gstride);
+    else
+        hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, m->stream, g, in, cin, taps, W, kpad, bias, cout, raw,
+                           gout, gstride);
+}
+
+// forward from m->x0: fills z, h, A and curl
+void forward(tmpg_model* m, const Geo& g)
+{
+    const uint16_t* in = m->x0;
+    int cin = 4;
+    for (int l = 0; l < m->nlayers; ++l) {
+        DLayer& L = m->L[l];
+        const bool last = l == m->nlayers - 1;
+        launch_conv(m, g, in, cin, L.taps, L.wf, L.kpad, L.cout_pad, L.b, L.cout, last ? m->A : m->z[l],
+                    last ? nullptr : m->h[l], L.cout);
+        if (!last) { in = m->h[l]; cin = L.cout; }
+    }
+    const int64_t P = g.npix * g.batch;
+    hipLaunchKernelGGL(k_curl, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, m->stream, g, m->A, m->curl);
+}
+
+// backward from m->gcurl (dL/dcurl): accumulates gw, gb
+void backward(tmpg_model* m, const Geo& g)
+{
+    const int64_t P = g.npix * g.batch;
+    const int64_t chunk = 1024;
+    hipLaunchKernelGGL(k_curl_adj, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, m->stream, g, m->gcurl, m->dA);
+    // dZ of the last layer: BF16 [P][4]
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)cdiv(P * 4, 256)), dim3(256), 0, m->stream, P, m->dA, 3, m->dz, 4);
+    int dz_c = 4;
+    for (int l = m->nlayers - 1; l >= 0; --l) {
+        DLayer& L = m->L[l];
+        const uint16_t* in = (l == 0) ? m->x0 : m->h[l - 1];
+        const int cin = (l == 0) ? 4 : m->L[l - 1].cout;
+        // weight gradient (dz stored with stride dz_c; cout real channels)
+        hipLaunchKernelGGL(k_conv_wgrad, dim3((unsigned)cdiv(P, chunk)), dim3(256), 0, m->stream, g, m->dz, dz_c,
+                           L.cout, L.cout_pad, in, cin, L.taps, L.kpad, chunk, L.gw, L.gb);
+        if (l == 0) break;
+        // input gradient: conv of dz with the transposed, flipped weights
+        launch_conv(m, g, m->dz, dz_c, L.taps, L.wt, L.kpadT, L.cinT_pad, nullptr, L.cin, m->dh, nullptr, 0);
+        const int c = m->L[l - 1].cout;
+        hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)cdiv(P * c, 256)), dim3(256), 0, m->stream, P * c, m->dh,
+                           m->z[l - 1], m->dz);
+        dz_c = c;
+    }
+}
+
+int set_geometry(tmpg_model* m, Geo& g, int nx, int ny, int nz, int batch)
+{
+    g.dim = m->dim;
+    g.nx = nx; g.ny = ny; g.nz = (m->dim == 3) ? nz : 1;
+    g.npix = (int64_t)g.nx * g.ny * g.nz;
+    g.batch = batch;
+    if (nx < 3 || ny < 3 || (m->dim == 3 && nz < 3) || batch < 1) { m->err = "bad denoiser geometry"; return TMPG_EINVAL; }
+    return ensure_work(m, g.npix * batch);
+}
+
+// forward + loss (+ backward) on device arrays jlo, jt [batch][3][n]
+int run_batch(tmpg_model* m, const Geo& g, const float* jlo, const float* jt, bool grad, double* loss)
+{
+    const int64_t n = g.npix, P = n * g.batch;
+    if (m->scale <= 0.0f) { m->err = "denoiser scale not set"; return TMPG_ESTATE; }
+    const float inv_s = 1.0f / m->scale;
+    hipLaunchKernelGGL(k_pack_input, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, m->stream, jlo, n, g.batch, inv_s,
+                       m->x0);
+    forward(m, g);
+    DN_CHECK(hipMemsetAsync(m->dloss, 0, sizeof(double), m->stream));
+    hipLaunchKernelGGL(k_loss, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, m->stream, n, g.batch, jlo, jt, inv_s,
+                       m->curl, grad ? m->gcurl : nullptr, m->dloss, 1.0 / (double)(3 * P));
+    if (grad) {
+        for (auto& L : m->L) {
+            DN_CHECK(hipMemsetAsync(L.gw, 0, (size_t)L.cout * L.kpad * sizeof(float), m->stream));
+            DN_CHECK(hipMemsetAsync(L.gb, 0, (size_t)L.cout * sizeof(float), m->stream));
+        }
+        backward(m, g);
+    }
+    DN_CHECK(hipGetLastError());
+    if (loss) {
+        DN_CHECK(hipMemcpyAsync(loss, m->dloss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        DN_CHECK(hipStreamSynchronize(m->stream));
+    }
+    return TMPG_OK;
+}
+
+int adam_step(tmpg_model* m, float lr)
+{
+    m->adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f - std::pow(b1, (float)m->adam_t), c2 = 1.0f - std::pow(b2, (float)m->adam_t);
+    for (auto& L : m->L) {
+        const int64_t nw = (int64_t)L.cout * L.kpad;
+        hipLaunchKernelGGL(k_adam, dim3((unsigned)cdiv(nw, 256)), dim3(256), 0, m->stream, nw, L.w, L.gw, L.m1w, L.m2w,
+                           lr, b1, b2, eps, c1, c2, (const uint8_t*)nullptr);
+        hipLaunchKernelGGL(k_adam, dim3((unsigned)cdiv(L.cout, 256)), dim3(256), 0, m->stream, (int64_t)L.cout, L.b,
+                           L.gb, L.m1b, L.m2b, lr, b1, b2, eps, c1, c2, (const uint8_t*)nullptr);
+    }
+    DN_CHECK(hipGetLastError());
+    return prep_weights(m);
+}
+
+// flat parameter order: per layer W[cout][taps*cin_real] (cin_real: 3 for
+// the first layer, else the previous width) then b[cout]
+int64_t nparams(const tmpg_model* m)
+{
+    int64_t n = 0;
+    for (size_t l = 0; l < m->L.size(); ++l) {
+        const DLayer& L = m->L[l];
+        const int cr = (l == 0) ? 3 : L.cin;
+        n += (int64_t)L.cout * L.taps * cr + L.cout;
+    }
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tmpg_model_create(int dim, uint64_t seed, tmpg_model** out)
+{
+    if (!out || (dim != 2 && dim != 3)) return TMPG_EINVAL;
+    tmpg_model* m = new (std::nothrow) tmpg_model();
+    if (!m) return TMPG_ENOMEM;
+    m->dim = dim;
+    m->width = (dim == 2) ? 32 : 16;
+    m->nlayers = (dim == 2) ? 4 : 3;
+    const int taps = (dim == 2) ? 9 : 27;
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { delete m; return TMPG_EHIP; }
+    uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto rnd = [&]() {  // uniform (-1,1), LCG + xorshift
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        uint64_t x = st ^ (st >> 33);
+        return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    };
+    int rc = 0;
+    for (int l = 0; l < m->nlayers; ++l) {
+        DLayer L;
+        L.cin = (l == 0) ? 4 : m->width;
+        L.cout = (l == m->nlayers - 1) ? 3 : m->width;
+        L.cout_pad = (L.cout + 15) / 16 * 16;
+        L.taps = taps;
+        L.kpad = (taps * L.cin + 15) / 16 * 16;
+        L.cinT = (L.cout + 3) / 4 * 4;
+        L.cinT_pad = (L.cin + 15) / 16 * 16;
+        L.kpadT = (taps * L.cinT + 15) / 16 * 16;
+        const int64_t nw = (int64_t)L.cout * L.kpad;
+        rc |= dn_alloc(m, &L.w, nw); rc |= dn_alloc(m, &L.b, L.cout);
+        rc |= dn_alloc(m, &L.gw, nw); rc |= dn_alloc(m, &L.gb, L.cout);
+        rc |= dn_alloc(m, &L.m1w, nw); rc |= dn_alloc(m, &L.m2w, nw);
+        rc |= dn_alloc(m, &L.m1b, L.cout); rc |= dn_alloc(m, &L.m2b, L.cout);
+        rc |= dn_alloc(m, &L.wf, (int64_t)L.cout_pad * L.kpad);
+        rc |= dn_alloc(m, &L.wt, (int64_t)L.cinT_pad * L.kpadT);
+        if (rc) break;
+        // He-uniform init over the real fan-in; the last layer starts small
+        // so the untrained correction is close to zero
+        const int cr = (l == 0) ? 3 : L.cin;
+        const double a = std::sqrt(6.0 / (taps * cr)) * ((l == m->nlayers - 1) ? 0.01 : 1.0);
+        std::vector<float> hw((size_t)nw, 0.0f);
+        for (int co = 0; co < L.cout; ++co)
+            for (int t = 0; t < taps; ++t)
+                for (int ci = 0; ci < cr; ++ci) hw[(size_t)co * L.kpad + t * L.cin + ci] = (float)(a * rnd());
+        std::vector<float> zero((size_t)nw, 0.0f);
+        (void)hipMemcpy(L.w, hw.data(), nw * sizeof(float), hipMemcpyHostToDevice);
+        (void)hipMemset(L.b, 0, L.cout * sizeof(float));
+        (void)hipMemset(L.m1w, 0, nw * sizeof(float));
+        (void)hipMemset(L.m2w, 0, nw * sizeof(float));
+        (void)hipMemset(L.m1b, 0, L.cout * sizeof(float));
+        (void)hipMemset(L.m2b, 0, L.cout * sizeof(float));
+        m->L.push_back(L);
+    }
+    rc |= dn_alloc(m, &m->dloss, 1);
+    if (rc || prep_weights(m)) { tmpg_model_destroy(m); return TMPG_ENOMEM; }
+    (void)hipStreamSynchronize(m->stream);
+    *out = m;
+    return TMPG_OK;
+}
+
+int tmpg_model_destroy(tmpg_model* m)
+{
+    if (!m) return TMPG_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    free_work(m);
+    auto f = [](void* p) { if (p) (void)hipFree(p); };
+    for (auto& L : m->L) { f(L.w); f(L.b); f(L.gw); f(L.gb); f(L.m1w); f(L.m2w); f(L.m1b); f(L.m2b); f(L.wf); f(L.wt); }
+    for (auto p : m->set_lo) f(p);
+    for (auto p : m->set_t) f(p);
+    f(m->dloss);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return TMPG_OK;
+}
+
+const char* tmpg_model_last_error(const tmpg_model* m) { return m ? m->err.c_str() : ""; }
+
+int tmpg_model_num_params(const tmpg_model* m, int64_t* n)
+{
+    if (!m || !n) return TMPG_EINVAL;
+    *n = nparams(m);
+    return TMPG_OK;
+}
+
+static int params_io(tmpg_model* m, float* host, bool get, bool grads)
+{
+    (void)hipStreamSynchronize(m->stream);
+    int64_t off = 0;
+    for (size_t l = 0; l < m->L.size(); ++l) {
+        DLayer& L = m->L[l];
+        const int cr = (l == 0) ? 3 : L.cin;
+        const int64_t nw = (int64_t)L.cout * L.kpad;
+        std::vector<float> w((size_t)nw), b((size_t)L.cout);
+        float* dw = grads ? L.gw : L.w;
+        float* dbv = grads ? L.gb : L.b;
+        if (get || true) {
+            DN_CHECK(hipMemcpy(w.data(), dw, nw * sizeof(float), hipMemcpyDeviceToHost));
+            DN_CHECK(hipMemcpy(b.data(), dbv, L.cout * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        for (int co = 0; co < L.cout; ++co)
+            for (int t = 0; t < L.taps; ++t)
+                for (int ci = 0; ci < cr; ++ci) {
+                    float& v = w[(size_t)co * L.kpad + t * L.cin + ci];
+                    if (get) host[off] = v; else v = host[off];
+                    ++off;
+                }
+        for (int co = 0; co < L.cout; ++co) {
+            if (get) host[off] = b[co]; else b[co] = host[off];
+            ++off;
+        }
+        if (!get) {
+            DN_CHECK(hipMemcpy(dw, w.data(), nw * sizeof(float), hipMemcpyHostToDevice));
+            DN_CHECK(hipMemcpy(dbv, b.data(), L.cout * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+    if (!get && !grads) return prep_weights(m);
+    return TMPG_OK;
+}
+
+int tmpg_model_get_params(tmpg_model* m, float* out) { return (m && out) ? params_io(m, out, true, false) : TMPG_EINVAL; }
+int tmpg_model_set_params(tmpg_model* m, const float* in)
+{
+    return (m && in) ? params_io(m, const_cast<float*>(in), false, false) : TMPG_EINVAL;
+}
+
+int tmpg_model_set_scale(tmpg_model* m, float s)
+{
+    if (!m || !(s > 0.0f)) return TMPG_EINVAL;
+    m->scale = s;
+    return TMPG_OK;
+}
+
+int tmpg_model_get_scale(const tmpg_model* m, float* s)
+{
+    if (!m || !s) return TMPG_EINVAL;
+    *s = m->scale;
+    return TMPG_OK;
+}
+
+// one batch from host arrays [batch][3][n]: loss, optional flat gradient,
+// optional Adam update with learning rate lr (update != 0)
+int tmpg_model_train_batch(tmpg_model* m, int nx, int ny, int nz, int batch, const float* jlo, const float* jt,
+                           float lr, int update, double* loss, float* grad_out)
+{
+    if (!m || !jlo || !jt) return TMPG_EINVAL;
+    Geo g;
+    int rc = set_geometry(m, g, nx, ny, nz, batch);
+    if (rc) return rc;
+    const size_t bytes = (size_t)g.npix * 3 * batch * sizeof(float);
+    DN_CHECK(hipMemcpyAsync(m->jlo, jlo, bytes, hipMemcpyHostToDevice, m->stream));
+    DN_CHECK(hipMemcpyAsync(m->jt, jt, bytes, hipMemcpyHostToDevice, m->stream));
+    rc = run_batch(m, g, m->jlo, m->jt, true, loss);
+    if (rc) return rc;
+    if (grad_out) {
+        rc = params_io(m, grad_out, true, true);
+        if (rc) return rc;
+    }
+    if (update) rc = adam_step(m, lr);
+    return rc;
+}
+
+// loss only, and optionally the corrected current J + s*curl(A) [batch][3][n]
+int tmpg_model_eval(tmpg_model* m, int nx, int ny, int nz, int batch, const float* jlo, const float* jt, double* loss,
+                    float* jout)
+{
+    if (!m || !jlo || !jt) return TMPG_EINVAL;
+    Geo g;
+    int rc = set_geometry(m, g, nx, ny, nz, batch);
+    if (rc) return rc;
+    const size_t bytes = (size_t)g.npix * 3 * batch * sizeof(float);
+    DN_CHECK(hipMemcpyAsync(m->jlo, jlo, bytes, hipMemcpyHostToDevice, m->stream));
+    DN_CHECK(hipMemcpyAsync(m->jt, jt, bytes, hipMemcpyHostToDevice, m->stream));
+    rc = run_batch(m, g, m->jlo, m->jt, false, loss);
+    if (rc) return rc;
+    if (jout) {
+        std::vector<float> c((size_t)g.npix * 3 * batch);
+        DN_CHECK(hipMemcpy(c.data(), m->curl, c.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int b = 0; b < batch; ++b)
+            for (int q = 0; q < 3; ++q)
+                for (int64_t p = 0; p < g.npix; ++p) {
+                    const size_t o = ((size_t)b * 3 + q) * g.npix + p;
+                    jout[o] = jlo[o] + m->scale * c[((size_t)b * g.npix + p) * 3 + q];
+                }
+    }
+    return TMPG_OK;
+}
+
+// add one training pair (host arrays [3][n]) to the device-resident set; the
+// first pair sets the input scale to the rms of its low-ppc current
+int tmpg_model_add_sample(tmpg_model* m, int nx, int ny, int nz, const float* jlo, const float* jt)
+{
+    if (!m || !jlo || !jt) return TMPG_EINVAL;
+    const int nzz = (m->dim == 3) ? nz : 1;
+    if (!m->set_lo.empty() && (nx != m->set_nx || ny != m->set_ny || nzz != m->set_nz)) {
+        m->err = "training samples must share one grid";
+        return TMPG_EINVAL;
+    }
+    m->set_nx = nx; m->set_ny = ny; m->set_nz = nzz;
+    const int64_t n = (int64_t)nx * ny * nzz * 3;
+    if (m->scale <= 0.0f) {
+        double s2 = 0.0;
+        for (int64_t i = 0; i < n; ++i) s2 += (double)jlo[i] * jlo[i];
+        m->scale = (float)std::sqrt(s2 / (double)n);
+        if (!(m->scale > 0.0f)) m->scale = 1.0f;
+    }
+    float *a, *b;
+    if (dn_alloc(m, &a, n) || dn_alloc(m, &b, n)) return TMPG_ENOMEM;
+    DN_CHECK(hipMemcpy(a, jlo, n * sizeof(float), hipMemcpyHostToDevice));
+    DN_CHECK(hipMemcpy(b, jt, n * sizeof(float), hipMemcpyHostToDevice));
+    m->set_lo.push_back(a);
+    m->set_t.push_back(b);
+    return TMPG_OK;
+}
+
+// iters Adam steps over the device-resident set, one sample per step in a
+// fixed shuffled order; losses[i] (optional) = loss of step i
+int tmpg_model_train(tmpg_model* m, int iters, float lr, uint64_t seed, double* losses)
+{
+    if (!m || iters < 0) return TMPG_EINVAL;
+    if (m->set_lo.empty()) { m->err = "no training samples"; return TMPG_ESTATE; }
+    Geo g;
+    int rc = set_geometry(m, g, m->set_nx, m->set_ny, m->set_nz, 1);
+    if (rc) return rc;
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    const size_t ns = m->set_lo.size();
+    for (int it = 0; it < iters; ++it) {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        const size_t k = (size_t)(st % ns);
+        double l = 0.0;
+        rc = run_batch(m, g, m->set_lo[k], m->set_t[k], true, losses ? &l : nullptr);
+        if (rc) return rc;
+        if (losses) losses[it] = l;
+        // cosine decay of the learning rate
+        const float f = 0.5f * (1.0f + std::cos(3.14159265f * (float)it / (float)std::max(iters, 1)));
+        rc = adam_step(m, lr * (0.05f + 0.95f * f));
+        if (rc) return rc;
+    }
+    DN_CHECK(hipStreamSynchronize(m->stream));
+    return TMPG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------- use inside the step loop
+namespace tmpg {
+int denoise_apply(tmpg_ctx* ctx, tmpg_model* m)
+{
+    Geo g;
+    int rc = set_geometry(m, g, ctx->g.nx, ctx->g.ny, ctx->g.nz, 1);
+    if (rc) { ctx->err = m->err; return rc; }
+    if (m->scale <= 0.0f) { ctx->err = "denoiser scale not set"; return TMPG_ESTATE; }
+    // the model's stream waits for the context's stream and vice versa
+    hipEvent_t e1, e2;
+    TMPG_HIP_CHECK(ctx, hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+    TMPG_HIP_CHECK(ctx, hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+    TMPG_HIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipStreamWaitEvent(m->stream, e1, 0));
+    const int64_t n = g.npix;
+    // J arrays are separate component arrays: pack them to [3][n] first
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(m->jlo, ctx->f[JX], n * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(m->jlo + n, ctx->f[JY], n * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(m->jlo + 2 * n, ctx->f[JZ], n * sizeof(float), hipMemcpyDeviceToDevice,
+                                       m->stream));
+    hipLaunchKernelGGL(k_pack_input, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, m->jlo, n, 1,
+                       1.0f / m->scale, m->x0);
+    forward(m, g);
+    hipLaunchKernelGGL(k_apply, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, n, m->curl, m->scale,
+                       ctx->f[JX], ctx->f[JY], ctx->f[JZ]);
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    TMPG_HIP_CHECK(ctx, hipEventRecord(e2, m->stream));
+    TMPG_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, e2, 0));
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    return TMPG_OK;
+}
+}  // namespace tmpg
+
+extern "C" {
+int tmpg_denoise(tmpg_ctx* ctx, tmpg_model* m)
+{
+    if (!ctx || !m) return TMPG_EINVAL;
+    if (m->dim != ctx->g.dim) { ctx->err = "denoiser dimensionality differs from the grid's"; return TMPG_EINVAL; }
+    return tmpg::denoise_apply(ctx, m);
+}
+
+int tmpg_set_denoiser(tmpg_ctx* ctx, tmpg_model* m)
+{
+    if (!ctx) return TMPG_EINVAL;
+    if (m && m->dim != ctx->g.dim) { ctx->err = "denoiser dimensionality differs from the grid's"; return TMPG_EINVAL; }
+    ctx->denoiser = m;
+    return TMPG_OK;
+}
+}  // extern "C"

@@ -528,7 +528,7 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
         (void)hipEventRecord(ev[2], ctx->stream);
         if (!rc) rc = fields_bhalf(ctx);
         (void)hipEventRecord(ev[3], ctx->stream);
-        if (!rc) rc = fields_filter(ctx, ctx->cfg.ntimes);
+        if (!rc) rc = ctx->denoiser ? denoise_apply(ctx, ctx->denoiser) : fields_filter(ctx, ctx->cfg.ntimes);
         (void)hipEventRecord(ev[4], ctx->stream);
         if (!rc) rc = fields_efull(ctx, true);
         (void)hipEventRecord(ev[5], ctx->stream);

@@ -98,6 +98,7 @@ struct tmpg_ctx {
     tmpg::Species sp[2];
     bool move_pending = false;
     bool use_cell_sums = true;  // 3D: per-cell register sums of the deposit
+    tmpg_model* denoiser = nullptr;  // replaces the filter passes when set
     // diagnostics (FP64)
     double* rho = nullptr;
     double* rho_prev = nullptr;
@@ -149,4 +150,5 @@ int particles_check(tmpg_ctx* ctx);
 int diag_energies(tmpg_ctx* ctx, double out[4]);
 int diag_rho(tmpg_ctx* ctx, double* rho_dev);
 int diag_continuity(tmpg_ctx* ctx, double* maxres, double* rhonorm);
+int denoise_apply(tmpg_ctx* ctx, tmpg_model* m);
 }  // namespace tmpg

@@ -33,6 +33,10 @@ EXPORTS = [
     "tmpg_diag", "tmpg_charge_density", "tmpg_track_continuity", "tmpg_load_uniform",
     "tmpg_last_timings", "tmpg_input_open", "tmpg_input_close", "tmpg_input_get_int",
     "tmpg_input_get_double",
+    "tmpg_model_create", "tmpg_model_destroy", "tmpg_model_last_error", "tmpg_model_num_params",
+    "tmpg_model_get_params", "tmpg_model_set_params", "tmpg_model_set_scale", "tmpg_model_get_scale",
+    "tmpg_model_train_batch", "tmpg_model_eval", "tmpg_model_add_sample", "tmpg_model_train",
+    "tmpg_denoise", "tmpg_set_denoiser",
 ]
 
 
@@ -73,6 +77,8 @@ def load_library(path=LIB_PATH):
         lib = ctypes.CDLL(path)
         lib.tmpg_last_error.restype = ctypes.c_char_p
         lib.tmpg_last_error.argtypes = [ctypes.c_void_p]
+        lib.tmpg_model_last_error.restype = ctypes.c_char_p
+        lib.tmpg_model_last_error.argtypes = [ctypes.c_void_p]
         _lib = lib
     return _lib
 
@@ -233,3 +239,106 @@ class Context:
         out = np.zeros(5, dtype=np.float64)
         self._chk(self.lib.tmpg_last_timings(self.h, _p(out)), "last_timings")
         return out
+
+
+class Model:
+    """The learned current denoiser (tmpg_model): a periodic CNN whose output
+    A gives the correction J += s * curl_b(A).  Current arrays here are the
+    periodic interior, shape (batch, 3, nz, ny, nx) or (3, nz, ny, nx)."""
+
+    def __init__(self, dim, seed=0):
+        self.lib = load_library()
+        self.dim = dim
+        h = ctypes.c_void_p()
+        rc = self.lib.tmpg_model_create(dim, ctypes.c_uint64(seed), ctypes.byref(h))
+        if rc != 0:
+            raise TmpgError(f"tmpg_model_create: {rc}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise TmpgError(f"{what}: {rc}: {self.lib.tmpg_model_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tmpg_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nparams(self):
+        n = ctypes.c_int64()
+        self._chk(self.lib.tmpg_model_num_params(self.h, ctypes.byref(n)), "num_params")
+        return n.value
+
+    def get_params(self):
+        out = np.zeros(self.nparams(), dtype=np.float32)
+        self._chk(self.lib.tmpg_model_get_params(self.h, _p(out)), "get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams()
+        self._chk(self.lib.tmpg_model_set_params(self.h, _p(p)), "set_params")
+
+    def set_scale(self, s):
+        self._chk(self.lib.tmpg_model_set_scale(self.h, ctypes.c_float(s)), "set_scale")
+
+    def scale(self):
+        s = ctypes.c_float()
+        self._chk(self.lib.tmpg_model_get_scale(self.h, ctypes.byref(s)), "get_scale")
+        return s.value
+
+    @staticmethod
+    def _batch(j):
+        j = np.ascontiguousarray(j, dtype=np.float32)
+        if j.ndim == 4:
+            j = j[None]
+        return j
+
+    def train_batch(self, jlo, jt, lr=0.0, update=False, want_grad=True):
+        a, b = self._batch(jlo), self._batch(jt)
+        bsz, _, nz, ny, nx = a.shape
+        loss = ctypes.c_double()
+        g = np.zeros(self.nparams(), dtype=np.float32) if want_grad else None
+        self._chk(self.lib.tmpg_model_train_batch(self.h, nx, ny, nz, bsz, _p(a), _p(b), ctypes.c_float(lr),
+                                                  int(bool(update)), ctypes.byref(loss), _p(g)), "train_batch")
+        return loss.value, g
+
+    def eval(self, jlo, jt, want_j=False):
+        a, b = self._batch(jlo), self._batch(jt)
+        bsz, _, nz, ny, nx = a.shape
+        loss = ctypes.c_double()
+        out = np.zeros_like(a) if want_j else None
+        self._chk(self.lib.tmpg_model_eval(self.h, nx, ny, nz, bsz, _p(a), _p(b), ctypes.byref(loss), _p(out)),
+                  "eval")
+        return (loss.value, out) if want_j else loss.value
+
+    def add_sample(self, jlo, jt):
+        a = np.ascontiguousarray(jlo, dtype=np.float32)
+        b = np.ascontiguousarray(jt, dtype=np.float32)
+        _, nz, ny, nx = a.shape
+        self._chk(self.lib.tmpg_model_add_sample(self.h, nx, ny, nz, _p(a), _p(b)), "add_sample")
+
+    def train(self, iters, lr=1e-3, seed=0):
+        losses = np.zeros(iters, dtype=np.float64)
+        self._chk(self.lib.tmpg_model_train(self.h, iters, ctypes.c_float(lr), ctypes.c_uint64(seed), _p(losses)),
+                  "train")
+        return losses
+
+
+def _ctx_denoise(self, model):
+    self._chk(self.lib.tmpg_denoise(self.h, model.h), "denoise")
+
+
+def _ctx_set_denoiser(self, model):
+    self._chk(self.lib.tmpg_set_denoiser(self.h, None if model is None else model.h), "set_denoiser")
+    self._denoiser = model   # keep the model alive while attached
+
+
+Context.denoise = _ctx_denoise
+Context.set_denoiser = _ctx_set_denoiser
