@@ -146,14 +146,18 @@ int tmpg_load_uniform(tmpg_ctx* ctx, int species, int ppc, float drift_gb,
  * [2] filter or denoiser, [3] number of steps summed, [4] total */
 int tmpg_last_timings(tmpg_ctx* ctx, double ms[5]);
 
-/* learned current denoiser (no reference counterpart: replaces the
- * apply_filter1_opt passes in low particles-per-cell runs) ---------------
+/* learned current denoiser (no reference counterpart: takes the place of
+ * the apply_filter1_opt passes in low particles-per-cell runs) ------------
  * A periodic CNN (2D: 4 layers 3x3, width 32; 3D: 3 layers 3x3x3, width 16,
  * GELU) maps J/s to a vector potential A on the B grid; the correction is
  * J += s*curl_b(A) with the E update's backward-difference curl, so it adds
- * no divergence.  BF16 MFMA convolutions with FP32 accumulation; training
- * (Adam) on the device.  Current arrays at this interface are the periodic
- * interior, [batch][3][nx*ny*nz] floats, x fastest. */
+ * no divergence.  With base_passes = n > 0 the model acts on the current
+ * after n binomial passes (J <- F^n J, then the correction); with n = 0 it
+ * acts on the raw deposit and replaces the filter entirely.  BF16 MFMA
+ * convolutions with FP32 accumulation; training (Adam) on the device.
+ * Current arrays at this interface are the periodic interior,
+ * [batch][3][nx*ny*nz] floats, x fastest; training pairs are given as the
+ * model's input (already filtered when base_passes > 0) and its target. */
 typedef struct tmpg_model tmpg_model;
 int tmpg_model_create(int dim, uint64_t seed, tmpg_model** out);
 int tmpg_model_destroy(tmpg_model* m);
@@ -164,6 +168,8 @@ int tmpg_model_get_params(tmpg_model* m, float* out);
 int tmpg_model_set_params(tmpg_model* m, const float* in);
 int tmpg_model_set_scale(tmpg_model* m, float s);
 int tmpg_model_get_scale(const tmpg_model* m, float* s);
+int tmpg_model_set_base_passes(tmpg_model* m, int npasses);
+int tmpg_model_get_base_passes(const tmpg_model* m, int* npasses);
 /* one forward/backward on a batch: loss = mean((J_lo/s + curl A - J_t/s)^2);
  * grad_out (may be NULL) receives the flat gradient; update != 0 applies Adam */
 int tmpg_model_train_batch(tmpg_model* m, int nx, int ny, int nz, int batch, const float* jlo,
@@ -173,8 +179,8 @@ int tmpg_model_eval(tmpg_model* m, int nx, int ny, int nz, int batch, const floa
 /* device-resident training set and Adam loop (cosine learning-rate decay) */
 int tmpg_model_add_sample(tmpg_model* m, int nx, int ny, int nz, const float* jlo, const float* jt);
 int tmpg_model_train(tmpg_model* m, int iters, float lr, uint64_t seed, double* losses);
-/* apply to the context's current now / use instead of the filter in tmpg_step
- * (NULL restores the filter) */
+/* apply to the context's current now (base passes, then the correction) /
+ * use instead of the filter in tmpg_step (NULL restores the filter) */
 int tmpg_denoise(tmpg_ctx* ctx, tmpg_model* m);
 int tmpg_set_denoiser(tmpg_ctx* ctx, tmpg_model* m);
 

@@ -57,13 +57,20 @@ def test_loss_and_gradients_match_oracle(tmpg, dim, shape, batch):
         go += g_ / batch
     assert abs(loss - lo) < 1e-4 * lo, (loss, lo)
     o = 0
-    for (c, t, i) in net.shapes:
+    for li, (c, t, i) in enumerate(net.shapes):
         for n in (c * t * i, c):
             a, b = g[o:o + n].astype(np.float64), go[o:o + n]
             o += n
+            if li == len(net.shapes) - 1 and n == c:
+                # the last layer's bias is a constant A, whose curl is zero:
+                # its gradient vanishes in exact arithmetic and is left with
+                # BF16 rounding residue only
+                wn = np.linalg.norm(go[o - n - c * t * i:o - n])
+                assert np.linalg.norm(a) < 1e-3 * wn and np.linalg.norm(b) < 1e-3 * wn, (a, b, wn)
+                continue
             err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
             cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
-            assert err < 2e-2 and cos > 0.9995, (n, err, cos)
+            assert err < 2e-2 and cos > 0.9995, (li, n, err, cos)
 
 
 def test_training_lowers_loss(tmpg):

@@ -213,6 +213,18 @@ __global__ void k_pack_input(const float* __restrict__ j, int64_t n, int batch, 
     x[t * 4 + 3] = 0;
 }
 
+// the same from the context's three current arrays (one image)
+__global__ void k_pack_input3(const float* __restrict__ jx, const float* __restrict__ jy,
+                              const float* __restrict__ jz, int64_t n, float inv_s, uint16_t* __restrict__ x)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    x[t * 4 + 0] = f2bf(jx[t] * inv_s);
+    x[t * 4 + 1] = f2bf(jy[t] * inv_s);
+    x[t * 4 + 2] = f2bf(jz[t] * inv_s);
+    x[t * 4 + 3] = 0;
+}
+
 // backward-difference curl on the periodic grid: c = curl_b(A), A [P][3]
 __global__ void k_curl(Geo g, const float* __restrict__ A, float* __restrict__ c)
 {
@@ -392,6 +404,7 @@ struct tmpg_model {
     int dim, width, nlayers;
     std::vector<DLayer> L;
     float scale = 0.0f;
+    int base_passes = 0;    // binomial passes applied to J before the correction
     int adam_t = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -735,6 +748,20 @@ int tmpg_model_set_scale(tmpg_model* m, float s)
     return TMPG_OK;
 }
 
+int tmpg_model_set_base_passes(tmpg_model* m, int npasses)
+{
+    if (!m || npasses < 0) return TMPG_EINVAL;
+    m->base_passes = npasses;
+    return TMPG_OK;
+}
+
+int tmpg_model_get_base_passes(const tmpg_model* m, int* npasses)
+{
+    if (!m || !npasses) return TMPG_EINVAL;
+    *npasses = m->base_passes;
+    return TMPG_OK;
+}
+
 int tmpg_model_get_scale(const tmpg_model* m, float* s)
 {
     if (!m || !s) return TMPG_EINVAL;
@@ -850,6 +877,10 @@ int tmpg_model_train(tmpg_model* m, int iters, float lr, uint64_t seed, double* 
 namespace tmpg {
 int denoise_apply(tmpg_ctx* ctx, tmpg_model* m)
 {
+    if (m->base_passes > 0) {
+        const int rc = fields_filter(ctx, m->base_passes);
+        if (rc) return rc;
+    }
     Geo g;
     int rc = set_geometry(m, g, ctx->g.nx, ctx->g.ny, ctx->g.nz, 1);
     if (rc) { ctx->err = m->err; return rc; }
@@ -861,13 +892,8 @@ int denoise_apply(tmpg_ctx* ctx, tmpg_model* m)
     TMPG_HIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
     TMPG_HIP_CHECK(ctx, hipStreamWaitEvent(m->stream, e1, 0));
     const int64_t n = g.npix;
-    // J arrays are separate component arrays: pack them to [3][n] first
-    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(m->jlo, ctx->f[JX], n * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
-    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(m->jlo + n, ctx->f[JY], n * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
-    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(m->jlo + 2 * n, ctx->f[JZ], n * sizeof(float), hipMemcpyDeviceToDevice,
-                                       m->stream));
-    hipLaunchKernelGGL(k_pack_input, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, m->jlo, n, 1,
-                       1.0f / m->scale, m->x0);
+    hipLaunchKernelGGL(k_pack_input3, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, ctx->f[JX],
+                       ctx->f[JY], ctx->f[JZ], n, 1.0f / m->scale, m->x0);
     forward(m, g);
     hipLaunchKernelGGL(k_apply, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, n, m->curl, m->scale,
                        ctx->f[JX], ctx->f[JY], ctx->f[JZ]);

@@ -21,6 +21,7 @@
 // reference recomputes the old position from the new momentum
 // (x0 = x - u c/gamma), which agrees in exact arithmetic.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "pic_push.h"
@@ -155,13 +156,24 @@ __global__ void __launch_bounds__(512) k_push_tiled(TiledArgs a)
         const int gj = wrap1(j0 - a.fo + lj, a.ny);
         const int gk = (DIM == 3) ? wrap1(k0 - a.fo + lk, a.nz) : 0;
         const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
+        if (DIM == 3) {
+            // node-major: six components per node
+            float2* d = reinterpret_cast<float2*>(sf) + 3 * e;
+            d[0] = make_float2(a.f6[0][g], a.f6[1][g]);
+            d[1] = make_float2(a.f6[2][g], a.f6[3][g]);
+            d[2] = make_float2(a.f6[4][g], a.f6[5][g]);
+        } else {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) sf[q * fplane + e] = a.f6[q][g];
+            for (int q = 0; q < 6; ++q) sf[q * fplane + e] = a.f6[q][g];
+        }
     }
     for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
     __syncthreads();
 
-    const LdsSrc src{sf, a.FX, a.FY, fplane};
+    using Src = typename std::conditional<DIM == 3, LdsSrc6, LdsSrc>::type;
+    Src src;
+    if constexpr (DIM == 3) src = LdsSrc6{sf, a.FX, a.FY};
+    else src = LdsSrc{sf, a.FX, a.FY, fplane};
     const LdsSink snk{sj, a.JX, a.JY, jplane};
     const int lane = __lane_id();
     const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -327,15 +339,17 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         const int gj = wrap1(j0 - a.fo + lj, a.ny);
         const int gk = wrap1(k0 - a.fo + lk, a.nz);
         const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) sf[q * fplane + e] = a.f6[q][g];
+        float2* d = reinterpret_cast<float2*>(sf) + 3 * e;   // node-major
+        d[0] = make_float2(a.f6[0][g], a.f6[1][g]);
+        d[1] = make_float2(a.f6[2][g], a.f6[3][g]);
+        d[2] = make_float2(a.f6[4][g], a.f6[5][g]);
     }
     for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
     for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
     if (tid == 0) *nrec = 0u;
     __syncthreads();
 
-    const LdsSrc src{sf, a.FX, a.FY, fplane};
+    const LdsSrc6 src{sf, a.FX, a.FY};
     const LdsSink snk{sj, a.JX, a.JY, jplane};
     const int lane = __lane_id();
     const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));

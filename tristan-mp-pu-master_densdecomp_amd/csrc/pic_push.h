@@ -27,6 +27,27 @@ struct GlobalSrc {
     }
     __device__ __forceinline__ int col(int i) const { return wrap1(i, nx); }
     __device__ __forceinline__ float get(int q, int64_t r, int c) const { return f[q][r + c]; }
+    __device__ __forceinline__ void get6(int64_t r, int c, float* v) const
+    {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[q] = f[q][r + c];
+    }
+};
+
+// 3D primal fields staged in LDS node-major: the six components of a node are
+// adjacent (24 bytes), so one node is three 8-byte LDS reads instead of six
+// 4-byte reads from six planes.  All six use the same primal weights.
+struct LdsSrc6 {
+    const float* s;     // [node][6], node = i + fx*(j + fy*k)
+    int fx, fy;
+    __device__ __forceinline__ int row(int j, int k) const { return (j + fy * k) * fx; }
+    __device__ __forceinline__ int col(int i) const { return i; }
+    __device__ __forceinline__ void get6(int r, int c, float* v) const
+    {
+        const float2* p = reinterpret_cast<const float2*>(s) + 3 * (r + c);
+        const float2 a = p[0], b = p[1], d = p[2];
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = d.x; v[5] = d.y;
+    }
 };
 
 struct LdsSrc {
@@ -35,6 +56,11 @@ struct LdsSrc {
     __device__ __forceinline__ int row(int j, int k) const { return (j + fy * k) * fx; }
     __device__ __forceinline__ int col(int i) const { return i; }
     __device__ __forceinline__ float get(int q, int r, int c) const { return s[q * plane + r + c]; }
+    __device__ __forceinline__ void get6(int r, int c, float* v) const
+    {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[q] = s[q * plane + r + c];
+    }
 };
 
 struct GlobalSink {
@@ -76,7 +102,7 @@ __device__ __forceinline__ void gather(const Src& src, bool quirk2, int i, int j
     constexpr int N0 = GatherRange<ORD>::N0, N1 = GatherRange<ORD>::N1;
 #pragma unroll
     for (int q = 0; q < 6; ++q) E[q] = 0.0f;
-    if (DIM == 3) {
+    if constexpr (DIM == 3) {
         float Spz[4], Sdz[4];
         int diz;
         gather_weights<ORD>(oz, quirk2, Spz, Sdz, diz);
@@ -90,9 +116,12 @@ __device__ __forceinline__ void gather(const Src& src, bool quirk2, int i, int j
                 const auto r = src.row(j - 1 + b, k - 1 + c);
                 float acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-                for (int t = N0; t < N1; ++t)
+                for (int t = N0; t < N1; ++t) {
+                    float v[6];
+                    src.get6(r, cx[t], v);
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) acc[q] += src.get(q, r, cx[t]) * Spx[t];
+                    for (int q = 0; q < 6; ++q) acc[q] += v[q] * Spx[t];
+                }
                 const float wyz = Spy[b] * Spz[c];
 #pragma unroll
                 for (int q = 0; q < 6; ++q) E[q] += acc[q] * wyz;
