@@ -10,10 +10,14 @@ library's loader; fields start at zero.
 A step is one full main-loop iteration (B half step, push+deposit, B half
 step, 4 filter passes, E step + current).  value = particles x steps / time.
 
-roofline: the push+deposit phase, priced at 56 B per particle (read+write of
-the packed cell, 3 offsets, 3 momenta) + 48 B per cell (6 field components
-read, 3 current components read-modify-write), per step, divided by the
-phase's average duration measured with HIP events on the library's stream.
+roofline: the tiled push+deposit kernel (k_push_cells in 3D), priced at 56 B
+per particle (read+write of the packed cell, 3 offsets, 3 momenta) + 48 B per
+cell (6 field components read, 3 current components read-modify-write) per
+launch (one launch per step), divided by the kernel's average duration
+measured with HIP events around its launch on the library's stream.
+traffic: HBM bytes per launch from FETCH_SIZE + WRITE_SIZE counter runs,
+scaled by the same counters on a plain copy of known size
+(profiles/traffic_<workload>.json).
 cpu_baseline: the reference's own Fortran routines (oracle/_ref, compiled
 from its sources) on one core over a bounded sample of the same workload
 (64 x 32 x 32 cells, same ppc and plasma), or the C++ oracle if that binary
@@ -181,7 +185,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="weibel3d_ord3_512x256x256_ppc4", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (5 times): the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, device="cuda:0", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one periodic domain per GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -201,8 +218,9 @@ def main():
     dt = t1 - t0
     value = npart * a.steps / dt
     push_ms = ms[1] / max(ms[3], 1)
+    kernel_ms = ms[5] / max(ms[3], 1)      # the tiled push+deposit kernel alone
     alg_bytes = 56.0 * npart + 48.0 * ncell
-    achieved = alg_bytes / (push_ms * 1e-3) / 1e9
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     traffic = read_traffic(a.workload)
     cpu = None
     if not a.no_cpu:
@@ -225,9 +243,12 @@ def main():
         "data": "synthetic (library loader: drifting Maxwell-Juttner beams, uniform positions)",
         "config": {"workload": a.workload, "grid": [w["nx"], w["ny"], w["nz"]], "order": w["order"],
                    "particles": npart, "ppc_per_species": w["ppc"], "filter_passes": w["ntimes"],
-                   "phase_ms": {"fields": ms[0] / ms[3], "push_deposit": push_ms, "filter": ms[2] / ms[3]}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                   "phase_ms": {"fields": ms[0] / ms[3], "push_deposit": push_ms, "push_kernel": kernel_ms,
+                                "filter": ms[2] / ms[3]}},
+        "roofline": {"bound": "hbm", "kernel": "k_push_cells" if w["dim"] == 3 else "k_push_tiled",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -143,8 +143,9 @@ int tmpg_load_uniform(tmpg_ctx* ctx, int species, int ppc, float drift_gb,
  * tmpg_step iterations since the previous call, per phase:
  * [0] field stencils (B half steps, E step + add current, J reset),
  * [1] push+deposit (including its field staging and particle re-sort),
- * [2] filter or denoiser, [3] number of steps summed, [4] total */
-int tmpg_last_timings(tmpg_ctx* ctx, double ms[5]);
+ * [2] filter or denoiser, [3] number of steps summed, [4] total,
+ * [5] the tiled push+deposit kernel alone (part of [1]) */
+int tmpg_last_timings(tmpg_ctx* ctx, double ms[6]);
 
 /* learned current denoiser (no reference counterpart: takes the place of
  * the apply_filter1_opt passes in low particles-per-cell runs) ------------

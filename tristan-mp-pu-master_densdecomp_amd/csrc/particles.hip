@@ -253,7 +253,7 @@ constexpr int CELLS_R = 2816;       // records per pass (both species)
 
 struct CellsLds {
     int fplane, jplane, nc;
-    size_t f_off, j_off, rec_off, rq_off, rc_off, idx_off, cnt_off, start_off, fill_off, bytes;
+    size_t f_off, j_off, rec_off, rq_off, rc_off, idx_off, perm_off, cnt_off, start_off, fill_off, bytes;
 };
 
 __host__ __device__ inline CellsLds cells_lds(int FX, int FY, int FZ, int JX, int JY, int JZ, int nc)
@@ -272,6 +272,7 @@ __host__ __device__ inline CellsLds cells_lds(int FX, int FY, int FZ, int JX, in
     L.fill_off = o; o += sizeof(int) * (size_t)nc;
     L.rc_off = o;  o += sizeof(uint16_t) * (size_t)CELLS_R;
     L.idx_off = o; o += sizeof(uint16_t) * (size_t)CELLS_R;
+    L.perm_off = o; o += sizeof(uint16_t) * (size_t)nc;
     o = (o + 15) & ~(size_t)15;
     o += 16;       // record counter
     L.bytes = o;
@@ -290,6 +291,62 @@ __device__ __forceinline__ void place(const float* v, int base, int first, float
 #pragma unroll
         for (int t = 0; t < W; ++t) acc = (s == base - first + t) ? v[t] : acc;
         out[s] = acc;
+    }
+}
+
+// Shape profile of one axis of a cell-summed record on a fixed node window:
+// S = old weights, dS = new - old.  profile6: nodes [-2,3] (x, cell
+// crossings included); profile4: nodes [-1,2] (y, z: the record's move keeps
+// the support inside the window).  Order 3 writes the pieces straight into
+// their slots: the old piece is the particle's own cell (m0 = 0), the new
+// one is m1 = floor(o + d) in {-1, 0, 1}, and a crossing splits dS at the
+// knot K = max(m0, m1) exactly as axis_deposit does.  Other orders go
+// through axis_deposit and place the window.
+template <int ORD>
+__device__ __forceinline__ void profile6(float o, float d, float* S6, float* dS6)
+{
+    if constexpr (ORD == 3) {
+        using P = Piece<3>;
+        float w[4], t0[4], t1[4];
+        P::w(o, w);
+        const int m1 = (int)floorf(o + d);
+        const float K = (m1 > 0) ? 1.0f : 0.0f;
+        const float b0 = (m1 == 0) ? o + d : K;
+        const float d0 = (m1 == 0) ? d : K - o;
+        P::dw(o, b0, d0, t0);
+        // second segment on piece m1 (length zero, so all zero, without a crossing)
+        const float d2 = d - d0;
+        const float a1 = K - (float)m1;
+        P::dw(a1, a1 + d2, d2, t1);
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            S6[u] = (u >= 1 && u <= 4) ? w[u - 1] : 0.0f;
+            const float v = (u >= 1 && u <= 4) ? t0[u - 1] : 0.0f;
+            const float up = (u >= 2) ? t1[u - 2] : 0.0f;    // m1 = +1: nodes 0..3
+            const float dn = (u <= 3) ? t1[u] : 0.0f;        // m1 = -1: nodes -2..1
+            dS6[u] = v + ((m1 > 0) ? up : dn);
+        }
+    } else {
+        constexpr int W = Win<ORD>::W;
+        float S1[W], dS[W];
+        const int b = axis_deposit<ORD>(o, d, S1, dS);
+        place<W, 6>(S1, b, -2, S6);
+        place<W, 6>(dS, b, -2, dS6);
+    }
+}
+
+template <int ORD>
+__device__ __forceinline__ void profile4(float o, float d, float* S4, float* dS4)
+{
+    if constexpr (ORD == 3) {
+        Piece<3>::w(o, S4);
+        Piece<3>::dw(o, o + d, d, dS4);
+    } else {
+        constexpr int W = Win<ORD>::W;
+        float S1[W], dS[W];
+        const int b = axis_deposit<ORD>(o, d, S1, dS);
+        place<W, 4>(S1, b, -1, S4);
+        place<W, 4>(dS, b, -1, dS4);
     }
 }
 
@@ -316,6 +373,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     int* fill = reinterpret_cast<int*>(smb + L.fill_off);
     uint16_t* rc = reinterpret_cast<uint16_t*>(smb + L.rc_off);
     uint16_t* idx = reinterpret_cast<uint16_t*>(smb + L.idx_off);
+    uint16_t* perm = reinterpret_cast<uint16_t*>(smb + L.perm_off);
     // record counts packed in one word: low 16 bits cell-summed records
     // (stored from 0 up), high 16 bits y/z crossers (stored from the top down)
     unsigned* nrec = reinterpret_cast<unsigned*>(smb + L.bytes - 16);
@@ -453,13 +511,24 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         const int pos = atomicAdd(fill + c, 1);
         idx[start[c] + pos] = (uint16_t)r;
     }
+    // cells in descending record count, so that the cells a wave sums
+    // together have similar loop lengths
+    for (int c = tid; c < nc; c += NT) {
+        const int n = cnt[c];
+        int rank = 0;
+        for (int d = 0; d < nc; ++d) {
+            const int m = cnt[d];
+            rank += (m > n) || (m == n && d < c);
+        }
+        perm[rank] = (uint16_t)c;
+    }
     __syncthreads();
 
     // ---- phase C: per-cell register sums, two threads per cell (z halves)
     constexpr int W = Win<ORD>::W;
     const float twelfth = 1.0f / 12.0f;
     for (int wk = tid; wk < 2 * nc; wk += NT) {
-        const int c = wk >> 1, half = wk & 1;
+        const int c = perm[wk >> 1], half = wk & 1;
         const int r0 = start[c], r1 = r0 + cnt[c];
         if (r1 == r0) continue;
         const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
@@ -484,29 +553,22 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             const float ox = rec[r], oy = rec[CELLS_R + r], oz = rec[2 * CELLS_R + r];
             const float dx = rec[3 * CELLS_R + r], dy = rec[4 * CELLS_R + r], dz = rec[5 * CELLS_R + r];
             const float q = rq[r];
-            float S1[W], dS[W];
             float dSx[6], Ax[6], dSy[4], Ay[4], dSz[4], Az[4];
             {
-                const int bx = axis_deposit<ORD>(ox, dx, S1, dS);
                 float S6[6];
-                place<W, 6>(S1, bx, -2, S6);
-                place<W, 6>(dS, bx, -2, dSx);
+                profile6<ORD>(ox, dx, S6, dSx);
 #pragma unroll
                 for (int u = 0; u < 6; ++u) Ax[u] = S6[u] + 0.5f * dSx[u];
             }
             {
-                const int by = axis_deposit<ORD>(oy, dy, S1, dS);
                 float S4[4];
-                place<W, 4>(S1, by, -1, S4);
-                place<W, 4>(dS, by, -1, dSy);
+                profile4<ORD>(oy, dy, S4, dSy);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) Ay[u] = S4[u] + 0.5f * dSy[u];
             }
             {
-                const int bz = axis_deposit<ORD>(oz, dz, S1, dS);
                 float S4[4];
-                place<W, 4>(S1, bz, -1, S4);
-                place<W, 4>(dS, bz, -1, dSz);
+                profile4<ORD>(oz, dz, S4, dSz);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) Az[u] = S4[u] + 0.5f * dSz[u];
             }
@@ -561,14 +623,14 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
                 const int row = snk.row(lj + 1 + b, lk + 1 + zs);
 #pragma unroll
                 for (int u = 0; u < 5; ++u)
-                    if (ax[u][b][h] != 0.0f) snk.add(0, row, li + u, ax[u][b][h]);
+                    snk.add(0, row, li + u, ax[u][b][h]);
             }
 #pragma unroll
             for (int b = 0; b < 3; ++b) {
                 const int row = snk.row(lj + 1 + b, lk + 1 + zs);
 #pragma unroll
                 for (int u = 0; u < 6; ++u)
-                    if (ay[u][b][h] != 0.0f) snk.add(1, row, li + u, ay[u][b][h]);
+                    snk.add(1, row, li + u, ay[u][b][h]);
             }
             if (half == 0 || h == 0) {
                 const int zsz = half ? 2 : h;
@@ -577,7 +639,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
                     const int row = snk.row(lj + 1 + b, lk + 1 + zsz);
 #pragma unroll
                     for (int u = 0; u < 6; ++u)
-                        if (az[u][b][h] != 0.0f) snk.add(2, row, li + u, az[u][b][h]);
+                        snk.add(2, row, li + u, az[u][b][h]);
                 }
             }
         }
@@ -1036,6 +1098,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
         const CellsLds CL = cells_lds(a.FX, a.FY, a.FZ, a.JX, a.JY, a.JZ, ncell_tile);
         const bool cells = g.dim == 3 && ctx->use_cell_sums && ncell_tile <= CELLS_T / 2 * 2 &&
                            CL.bytes <= 160 * 1024;
+        if (ctx->push_events) (void)hipEventRecord(ctx->push_events[0], ctx->stream);
         dispatch(g.dim, ctx->cfg.order, ch, id, [&](auto D, auto O, auto C, auto I) {
             constexpr int Dv = decltype(D)::value, Ov = decltype(O)::value;
             constexpr bool Cv = decltype(C)::value, Iv = decltype(I)::value;
@@ -1050,6 +1113,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
             }
         });
         TMPG_HIP_CHECK(ctx, hipGetLastError());
+        if (ctx->push_events) (void)hipEventRecord(ctx->push_events[1], ctx->stream);
         // overflow list of the previous step
         for (int s = 0; s < 2; ++s) {
             Species& sp = ctx->sp[s];

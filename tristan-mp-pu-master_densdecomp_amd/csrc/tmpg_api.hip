@@ -508,13 +508,13 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
             if (rc) break;
         }
         // per-phase HIP events on the context's stream, read by tmpg_last_timings
-        const size_t need = (size_t)(ctx->prof_steps + 1) * 6;
+        const size_t need = (size_t)(ctx->prof_steps + 1) * 8;
         while (ctx->prof.size() < need) {
             hipEvent_t e;
             TMPG_HIP_CHECK(ctx, hipEventCreate(&e));
             ctx->prof.push_back(e);
         }
-        hipEvent_t* ev = &ctx->prof[(size_t)ctx->prof_steps * 6];
+        hipEvent_t* ev = &ctx->prof[(size_t)ctx->prof_steps * 8];
         ctx->prof_steps++;
         // code/tristanmainloop.F90:117-242 with the ghost fills and the
         // current fold absorbed into periodic indexing, reset_currents moved
@@ -524,7 +524,9 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
         if (!rc) rc = fields_zero_j(ctx);
         (void)hipEventRecord(ev[1], ctx->stream);
         if (!rc) rc = pre_push(ctx);
+        ctx->push_events = ev + 6;   // around the tiled push kernel itself
         if (!rc) rc = particles_push(ctx, 0);
+        ctx->push_events = nullptr;
         (void)hipEventRecord(ev[2], ctx->stream);
         if (!rc) rc = fields_bhalf(ctx);
         (void)hipEventRecord(ev[3], ctx->stream);
@@ -537,23 +539,25 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
     return rc;
 }
 
-int tmpg_last_timings(tmpg_ctx* ctx, double ms[5])
+int tmpg_last_timings(tmpg_ctx* ctx, double ms[6])
 {
     if (!valid_ctx(ctx) || !ms) return TMPG_EINVAL;
     TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    for (int q = 0; q < 5; ++q) ms[q] = 0.0;
+    for (int q = 0; q < 6; ++q) ms[q] = 0.0;
     for (int s = 0; s < ctx->prof_steps; ++s) {
-        hipEvent_t* ev = &ctx->prof[(size_t)s * 6];
-        float t[5] = {0, 0, 0, 0, 0};
+        hipEvent_t* ev = &ctx->prof[(size_t)s * 8];
+        float t[6] = {0, 0, 0, 0, 0, 0};
         (void)hipEventElapsedTime(&t[0], ev[0], ev[1]);
         (void)hipEventElapsedTime(&t[1], ev[1], ev[2]);
         (void)hipEventElapsedTime(&t[2], ev[2], ev[3]);
         (void)hipEventElapsedTime(&t[3], ev[3], ev[4]);
         (void)hipEventElapsedTime(&t[4], ev[4], ev[5]);
+        (void)hipEventElapsedTime(&t[5], ev[6], ev[7]);
         ms[0] += (double)t[0] + t[2] + t[4];
         ms[1] += t[1];
         ms[2] += t[3];
         ms[4] += (double)t[0] + t[1] + t[2] + t[3] + t[4];
+        ms[5] += t[5];
     }
     ms[3] = (double)ctx->prof_steps;
     ctx->prof_steps = 0;

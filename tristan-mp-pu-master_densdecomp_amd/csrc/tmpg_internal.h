@@ -113,6 +113,7 @@ struct tmpg_ctx {
     hipEvent_t ev[8] = {};
     std::vector<hipEvent_t> prof;  // 6 events per tmpg_step iteration since the last read
     int prof_steps = 0;
+    hipEvent_t* push_events = nullptr;  // set during tmpg_step: recorded around the tiled push kernel
     std::string err;
 };
 

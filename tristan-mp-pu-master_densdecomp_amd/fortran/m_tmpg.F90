@@ -121,7 +121,7 @@ module m_tmpg
 		integer(c_int) function tmpg_last_timings(ctx, ms) bind(C, name="tmpg_last_timings")
 			import :: c_int, c_ptr, c_double
 			type(c_ptr), value :: ctx
-			real(c_double), intent(out) :: ms(5)
+			real(c_double), intent(out) :: ms(6)
 		end function
 		! input file in the reference's <block> key = value format
 		integer(c_int) function tmpg_input_open(path, inp) bind(C, name="tmpg_input_open")

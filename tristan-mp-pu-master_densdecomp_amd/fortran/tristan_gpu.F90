@@ -21,7 +21,7 @@ program tristan_gpu
 	type(c_ptr) :: ctx
 	character(len=512) :: infile, outfile, arg
 	integer :: lap, k, nargs, u
-	real(c_double) :: d(8), ms(5)
+	real(c_double) :: d(8), ms(6)
 	integer(c_int64_t) :: ni, ne
 	real(8) :: t0, t1, tacc
 	integer(8) :: c0, c1, crate

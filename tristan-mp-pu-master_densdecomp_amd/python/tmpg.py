@@ -237,7 +237,7 @@ class Context:
         self._chk(self.lib.tmpg_track_continuity(self.h, int(bool(on))), "track_continuity")
 
     def last_timings(self):
-        out = np.zeros(5, dtype=np.float64)
+        out = np.zeros(6, dtype=np.float64)
         self._chk(self.lib.tmpg_last_timings(self.h, _p(out)), "last_timings")
         return out
 
