@@ -141,7 +141,19 @@ def test_split_equals_fused(tmpg, dim, order):
 
 @pytest.mark.parametrize("dim", [2, 3])
 def test_filter_vs_oracle(tmpg, dim):
-    nx, ny, nz = (20, 18, 14) if dim == 3 else (64, 40, 1)
+    _filter_case(tmpg, dim, (20, 18, 14) if dim == 3 else (64, 40, 1), 4)
+
+
+@pytest.mark.parametrize("dim,shape,npasses", [(3, (130, 70, 40), 4), (3, (9, 7, 5), 3), (3, (12, 10, 8), 6),
+                                               (3, (66, 33, 70), 1), (2, (150, 97, 1), 2), (2, (5, 6, 1), 9)])
+def test_filter_sweeps_vs_oracle(tmpg, dim, shape, npasses):
+    """several workgroups and z chunks, grids smaller than the filter
+    support, pass counts split into sweeps of up to four"""
+    _filter_case(tmpg, dim, shape, npasses)
+
+
+def _filter_case(tmpg, dim, shape, npasses):
+    nx, ny, nz = shape
     order = 3
     cfg = orc.make_cfg(dim, order, nx, ny, nz)
     rng = np.random.default_rng(11)
@@ -150,10 +162,10 @@ def test_filter_vs_oracle(tmpg, dim):
     p = dict(c=0.45, corr=1.025, c_omp=4.0, gamma0=0.5, ppc0=4.0, me=1.0, mi=4.0)
     ctx = make_context(tmpg, dim, order, nx, ny, nz, p)
     ctx.set_currents(jp)
-    ctx.filter(4)
+    ctx.filter(npasses)
     got = _interior(ctx, ctx.get_currents())
     ref = jp.astype(np.float64).copy()
-    orc.filt(cfg, ref, 4)
+    orc.filt(cfg, ref, npasses)
     e = rel(got, ref[(slice(None),) + orc.interior_slices(cfg)])
     assert e < 1e-6, e
 

@@ -5,7 +5,8 @@
 // E step     : code/fields.F90:739-870 (advance_e_fullstep), optionally fused
 //              with add_current (code/fields.F90:1372-1395): E = (E + c*curl B) + J,
 //              the reference's two roundings in its order
-// filter     : code/filter.F90:8-221, (1,2,1)^d / 4^d per pass, periodic
+// filter     : code/filter.F90:8-221, (1,2,1)^d / 4^d per pass, periodic;
+//              up to four passes per sweep as the separable binomial
 // The reference fills ghost layers (bc_b1/bc_e1) and folds ghost currents;
 // here every neighbour index wraps periodically instead.
 #include "tmpg_internal.h"
@@ -97,48 +98,89 @@ __global__ void k_add(float* __restrict__ e, const float* __restrict__ j, int64_
 // Tile of 64 x 4 (x, y) cells per block, staged with a 1-cell halo in LDS for
 // the three z planes k-1, k, k+1; each output reads its 27 (9 in 2D)
 // neighbours from LDS.  The sum is the reference's (code/filter.F90:105-120).
-template <int DIM>
-__global__ void __launch_bounds__(256) k_filter(const float* __restrict__ in, float* __restrict__ out,
-                                                int nx, int ny, int nz)
+// NP binomial passes in one sweep.  On the periodic grid the pass is
+// (1,2,1)/4 along every axis, so NP passes are the separable 1D binomial
+// C(2NP,t)/4^NP, t = 0..2NP, along x, y and (3D) z.  A workgroup owns an
+// FT_X x FT_Y column of the grid over FT_Z planes: each plane (with an NP
+// halo) is staged in LDS and filtered along x then y; the z filter runs
+// over a ring of the last 2NP+1 xy-filtered planes held in registers.  One
+// read (plus halo) and one write per cell instead of NP of each.
+constexpr int FT_X = 64, FT_Y = 32, FT_Z = 32, FT_TY = 4;   // 64 x 4 threads, 8 rows each
+
+__device__ __forceinline__ int wrapany(int i, int n) { i %= n; return i < 0 ? i + n : i; }
+
+template <int NP> struct Binom;
+template <> struct Binom<1> { static constexpr float w[3] = {0.25f, 0.5f, 0.25f}; };
+template <> struct Binom<2> { static constexpr float w[5] = {1 / 16.f, 4 / 16.f, 6 / 16.f, 4 / 16.f, 1 / 16.f}; };
+template <> struct Binom<3> {
+    static constexpr float w[7] = {1 / 64.f, 6 / 64.f, 15 / 64.f, 20 / 64.f, 15 / 64.f, 6 / 64.f, 1 / 64.f};
+};
+template <> struct Binom<4> {
+    static constexpr float w[9] = {1 / 256.f,  8 / 256.f, 28 / 256.f, 56 / 256.f, 70 / 256.f,
+                                   56 / 256.f, 28 / 256.f, 8 / 256.f,  1 / 256.f};
+};
+
+template <int DIM, int NP>
+__global__ void __launch_bounds__(FT_X * FT_TY) k_filter_fused(const float* __restrict__ in, float* __restrict__ out,
+                                                               int nx, int ny, int nz)
 {
-    constexpr int NP = (DIM == 3) ? 3 : 1;
-    __shared__ float s[NP][BY_ + 2][BX_ + 2];
-    const int i0 = blockIdx.x * BX_, j0 = blockIdx.y * BY_, k = blockIdx.z;
-    const int tid = threadIdx.y * BX_ + threadIdx.x;
-    const int NT = BX_ * BY_;
-    constexpr int PLANE = (BY_ + 2) * (BX_ + 2);
-    for (int e = tid; e < NP * PLANE; e += NT) {
-        int p = e / PLANE, r = e - p * PLANE;
-        int ly = r / (BX_ + 2), lx = r - ly * (BX_ + 2);
-        int gi = wrapm(i0 + lx - 1, nx), gj = wrapm(j0 + ly - 1, ny);
-        int gk = (DIM == 3) ? wrapm(k + p - 1, nz) : 0;
-        s[p][ly][lx] = in[gi + (int64_t)nx * (gj + (int64_t)ny * gk)];
+    constexpr int T = 2 * NP + 1, RX = FT_X + 2 * NP, RY = FT_Y + 2 * NP, ROWS = FT_Y / FT_TY;
+    __shared__ float raw[RY][RX];
+    __shared__ float xs[RY][FT_X];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FT_X + tx;
+    const int i0 = blockIdx.x * FT_X, j0 = blockIdx.y * FT_Y;
+    const int k0 = (DIM == 3) ? blockIdx.z * FT_Z : 0;
+    const int k1 = (DIM == 3) ? min(k0 + FT_Z, nz) : 1;
+    const int i = i0 + tx;
+    float ring[ROWS][T] = {};
+    const int kfirst = (DIM == 3) ? k0 - NP : 0, klast = (DIM == 3) ? k1 - 1 + NP : 0;
+    for (int kk = kfirst; kk <= klast; ++kk) {
+        const int64_t plane = (int64_t)nx * ny * ((DIM == 3) ? wrapany(kk, nz) : 0);
+        for (int e = tid; e < RY * RX; e += FT_X * FT_TY) {
+            const int r = e / RX, c = e - r * RX;
+            raw[r][c] = in[plane + wrapany(i0 + c - NP, nx) + (int64_t)nx * wrapany(j0 + r - NP, ny)];
+        }
+        __syncthreads();
+        for (int r = ty; r < RY; r += FT_TY) {
+            float v = 0.0f;
+#pragma unroll
+            for (int t = 0; t < T; ++t) v += Binom<NP>::w[t] * raw[r][tx + t];
+            xs[r][tx] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+            const int r = ty + FT_TY * q;
+            float v = 0.0f;
+#pragma unroll
+            for (int t = 0; t < T; ++t) v += Binom<NP>::w[t] * xs[r + t][tx];
+            if (DIM == 3) {
+#pragma unroll
+                for (int t = 0; t + 1 < T; ++t) ring[q][t] = ring[q][t + 1];
+                ring[q][T - 1] = v;
+            } else {
+                ring[q][0] = v;
+            }
+        }
+        const int k = (DIM == 3) ? kk - NP : 0;
+        if ((DIM == 2 || kk - kfirst >= 2 * NP) && i < nx) {
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) {
+                const int j = j0 + ty + FT_TY * q;
+                if (j >= ny) continue;
+                float v;
+                if (DIM == 3) {
+                    v = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < T; ++t) v += Binom<NP>::w[t] * ring[q][t];
+                } else {
+                    v = ring[q][0];
+                }
+                out[i + (int64_t)nx * (j + (int64_t)ny * k)] = v;
+            }
+        }
+        __syncthreads();   // raw and xs are rewritten by the next plane
     }
-    __syncthreads();
-    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-    if (i >= nx || j >= ny) return;
-    const int x = threadIdx.x + 1, y = threadIdx.y + 1;
-    constexpr int c = (DIM == 3) ? 1 : 0;
-    float t;
-    if (DIM == 2) {
-        const float w = 1.0f / 16.0f;
-        const float wl = 2.0f * w, wt = 4.0f * w, wd = w;
-        t = wl * s[c][y][x - 1] + wt * s[c][y][x] + wl * s[c][y][x + 1] + wl * s[c][y - 1][x] +
-            wl * s[c][y + 1][x] + wd * s[c][y + 1][x - 1] + wd * s[c][y + 1][x + 1] +
-            wd * s[c][y - 1][x - 1] + wd * s[c][y - 1][x + 1];
-    } else {
-        const float w = 1.0f / 64.0f;
-        const float a = 4.0f * w, b = 8.0f * w, d = 2.0f * w, e = w;
-        t = a * s[1][y][x - 1] + b * s[1][y][x] + a * s[1][y][x + 1] + a * s[1][y - 1][x] +
-            a * s[1][y + 1][x] + d * s[1][y + 1][x - 1] + d * s[1][y + 1][x + 1] +
-            d * s[1][y - 1][x - 1] + d * s[1][y - 1][x + 1];
-        t = t + d * s[0][y][x - 1] + a * s[0][y][x] + d * s[0][y][x + 1] + d * s[0][y - 1][x] +
-            d * s[0][y + 1][x] + e * s[0][y + 1][x - 1] + e * s[0][y + 1][x + 1] +
-            e * s[0][y - 1][x - 1] + e * s[0][y - 1][x + 1] + d * s[2][y][x - 1] + a * s[2][y][x] +
-            d * s[2][y][x + 1] + d * s[2][y - 1][x] + d * s[2][y + 1][x] + e * s[2][y + 1][x - 1] +
-            e * s[2][y + 1][x + 1] + e * s[2][y - 1][x - 1] + e * s[2][y - 1][x + 1];
-    }
-    out[i + (int64_t)nx * (j + (int64_t)ny * k)] = t;
 }
 
 // primal-averaged fields for the 3D gather
@@ -228,15 +270,24 @@ int fields_add_current(tmpg_ctx* ctx)
 int fields_filter(tmpg_ctx* ctx, int npasses)
 {
     const Grid& g = ctx->g;
-    for (int p = 0; p < npasses; ++p) {
+    // npasses = 4a + b: a sweeps of four passes, then one of b
+    int left = npasses;
+    while (left > 0) {
+        const int np = left >= 4 ? 4 : left;
+        left -= np;
+        const dim3 grid((g.nx + FT_X - 1) / FT_X, (g.ny + FT_Y - 1) / FT_Y, g.dim == 3 ? (g.nz + FT_Z - 1) / FT_Z : 1);
         for (int q = 0; q < 3; ++q) {
             float* a = ctx->f[JX + q];
-            if (g.dim == 3)
-                hipLaunchKernelGGL(k_filter<3>, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, a, ctx->tmp, g.nx,
-                                   g.ny, g.nz);
-            else
-                hipLaunchKernelGGL(k_filter<2>, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, a, ctx->tmp, g.nx,
-                                   g.ny, g.nz);
+#define TMPG_FF(D, N) hipLaunchKernelGGL((k_filter_fused<D, N>), grid, dim3(FT_X, FT_TY), 0, ctx->stream, a, \
+                                         ctx->tmp, g.nx, g.ny, g.nz)
+            if (g.dim == 3) {
+                switch (np) { case 1: TMPG_FF(3, 1); break; case 2: TMPG_FF(3, 2); break;
+                              case 3: TMPG_FF(3, 3); break; default: TMPG_FF(3, 4); }
+            } else {
+                switch (np) { case 1: TMPG_FF(2, 1); break; case 2: TMPG_FF(2, 2); break;
+                              case 3: TMPG_FF(2, 3); break; default: TMPG_FF(2, 4); }
+            }
+#undef TMPG_FF
             // ping-pong: the filtered array becomes the current, the old one the scratch
             ctx->f[JX + q] = ctx->tmp;
             ctx->tmp = a;
