@@ -461,17 +461,17 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
                 }
                 td = tile_of(tl, p.i, p.j, p.k);
             }
+            // slots in b: leavers reserve one each in their new tile, the
+            // wave's stayers one block in this tile; both atomics are in
+            // flight together (one round trip)
             const bool stay = active && td == t;
             const uint64_t m = __ballot(stay);
-            int32_t slot = -1;
-            if (m) {
-                const int leader = __ffsll((unsigned long long)m) - 1;
-                int32_t b0 = 0;
-                if (lane == leader) b0 = atomicAdd(sp.cnt_b + t, __popcll(m));
-                b0 = __shfl(b0, leader, 64);
-                if (stay) slot = b0 + __popcll(m & ltmask);
-            }
-            if (active && !stay) slot = atomicAdd(sp.cnt_b + td, 1);
+            int32_t lslot = 0, b0 = 0;
+            if (active && !stay) lslot = atomicAdd(sp.cnt_b + td, 1);
+            const int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
+            if (m && lane == leader) b0 = atomicAdd(sp.cnt_b + t, __popcll(m));
+            b0 = __shfl(b0, leader, 64);
+            const int32_t slot = stay ? b0 + __popcll(m & ltmask) : lslot;
             if (active) emit<3, HAS_CH, HAS_ID>(sp, a.pk, slots, td, slot, p);
         }
     }
