@@ -253,7 +253,7 @@ constexpr int CELLS_R = 2816;       // records per pass (both species)
 
 struct CellsLds {
     int fplane, jplane, nc;
-    size_t f_off, j_off, rec_off, rq_off, rc_off, idx_off, perm_off, cnt_off, start_off, fill_off, bytes;
+    size_t f_off, j_off, rec_off, rq_off, rc_off, idx_off, cnt_off, start_off, fill_off, bytes;
 };
 
 __host__ __device__ inline CellsLds cells_lds(int FX, int FY, int FZ, int JX, int JY, int JZ, int nc)
@@ -272,7 +272,6 @@ __host__ __device__ inline CellsLds cells_lds(int FX, int FY, int FZ, int JX, in
     L.fill_off = o; o += sizeof(int) * (size_t)nc;
     L.rc_off = o;  o += sizeof(uint16_t) * (size_t)CELLS_R;
     L.idx_off = o; o += sizeof(uint16_t) * (size_t)CELLS_R;
-    L.perm_off = o; o += sizeof(uint16_t) * (size_t)nc;
     o = (o + 15) & ~(size_t)15;
     o += 16;       // record counter
     L.bytes = o;
@@ -373,7 +372,6 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     int* fill = reinterpret_cast<int*>(smb + L.fill_off);
     uint16_t* rc = reinterpret_cast<uint16_t*>(smb + L.rc_off);
     uint16_t* idx = reinterpret_cast<uint16_t*>(smb + L.idx_off);
-    uint16_t* perm = reinterpret_cast<uint16_t*>(smb + L.perm_off);
     // record counts packed in one word: low 16 bits cell-summed records
     // (stored from 0 up), high 16 bits y/z crossers (stored from the top down)
     unsigned* nrec = reinterpret_cast<unsigned*>(smb + L.bytes - 16);
@@ -419,31 +417,48 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         const int n = min(sp.cnt_a[t], sp.tcap);
         const int64_t base = t * sp.tcap;
         const int64_t slots = (int64_t)tl.ntiles * sp.tcap;
+        // the next round's particles are loaded while this round is pushed
+        Part pn;
+        bool an = tid < n;
+        if (an) load_part<3, HAS_CH, HAS_ID>(sp.a, base + tid, a.pk, pn);
         for (int p0 = 0; p0 < n; p0 += NT) {
-            const int pi = p0 + tid;
-            const bool active = pi < n;
-            Part p;
+            const bool active = an;
+            Part p = pn;
+            an = p0 + NT + tid < n;
+            if (an) load_part<3, HAS_CH, HAS_ID>(sp.a, base + p0 + NT + tid, a.pk, pn);
             int64_t td = -1;
+            float ox = 0.0f, oy = 0.0f, oz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f, q = 0.0f;
+            int li = 0, lj = 0, lk = 0;
+            bool fits = false;
             if (active) {
-                load_part<3, HAS_CH, HAS_ID>(sp.a, base + pi, a.pk, p);
-                const int li = p.i - i0, lj = p.j - j0, lk = p.k - k0;
-                const float ox = p.ox, oy = p.oy, oz = p.oz;
-                float dx, dy, dz;
+                li = p.i - i0; lj = p.j - j0; lk = p.k - k0;
+                ox = p.ox; oy = p.oy; oz = p.oz;
                 push_one<3, ORD>(src, a.quirk2, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx, dy,
                                  dz);
                 advance(p, 3, a.nx, a.ny, a.nz, dx, dy, dz);
-                const float q = sp.q * p.ch;
-                const bool fits = fits4<ORD>(oy, dy) && fits4<ORD>(oz, dz);
-                // cell-summed records grow from 0 up, y/z crossers from the
-                // top down; the two counts together never pass CELLS_R
-                int r = -1;
-                {
-                    const unsigned inc = fits ? 1u : 65536u;
-                    const unsigned old = atomicAdd(nrec, inc);
-                    const int lo = (int)(old & 0xffffu), hi = (int)(old >> 16);
-                    if (lo + hi < CELLS_R) r = fits ? lo : CELLS_R - 1 - hi;
-                    else atomicSub(nrec, inc);
-                }
+                q = sp.q * p.ch;
+                fits = fits4<ORD>(oy, dy) && fits4<ORD>(oz, dz);
+            }
+            // record slots, one LDS atomic per wave: cell-summed records grow
+            // from 0 up, y/z crossers from the top down.  A wave whose block
+            // would make the two meet takes its block back and deposits at
+            // once; any block reserved while such a failed one is pending
+            // fails too, so the reserved blocks stay contiguous.
+            int r = -1;
+            {
+                const uint64_t mf = __ballot(active && fits), mc = __ballot(active && !fits);
+                const unsigned inc = (unsigned)__popcll(mf) + ((unsigned)__popcll(mc) << 16);
+                unsigned old = 0;
+                if (inc && lane == 0) old = atomicAdd(nrec, inc);
+                old = __shfl(old, 0, 64);
+                const int lo = (int)(old & 0xffffu) + __popcll(mf), hi = (int)(old >> 16) + __popcll(mc);
+                const bool ok = lo + hi <= CELLS_R;
+                if (inc && !ok && lane == 0) atomicSub(nrec, inc);
+                if (ok && active)
+                    r = fits ? (int)(old & 0xffffu) + __popcll(mf & ltmask)
+                             : CELLS_R - 1 - ((int)(old >> 16) + __popcll(mc & ltmask));
+            }
+            if (active) {
                 const int c = li + tl.tx * (lj + tl.ty * lk);
                 if (r >= 0) {
                     rec[0 * CELLS_R + r] = ox;
@@ -477,10 +492,12 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     }
     __syncthreads();
 
-    // ---- dense deposit of the y/z crossers (all lanes busy)
+    // ---- direct deposit of the y/z crossers
     const int nr = (int)(*nrec & 0xffffu);
     const int nd = (int)(*nrec >> 16);
-    for (int k = tid; k < nd; k += NT) {
+    // crosser k goes to wave k % nwaves, so all waves share the pass
+    const int nwaves = NT >> 6;
+    for (int k = (tid & 63) * nwaves + (tid >> 6); k < nd; k += NT) {
         const int r = CELLS_R - 1 - k;
         const int c = rc[r];
         const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
@@ -511,24 +528,13 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         const int pos = atomicAdd(fill + c, 1);
         idx[start[c] + pos] = (uint16_t)r;
     }
-    // cells in descending record count, so that the cells a wave sums
-    // together have similar loop lengths
-    for (int c = tid; c < nc; c += NT) {
-        const int n = cnt[c];
-        int rank = 0;
-        for (int d = 0; d < nc; ++d) {
-            const int m = cnt[d];
-            rank += (m > n) || (m == n && d < c);
-        }
-        perm[rank] = (uint16_t)c;
-    }
     __syncthreads();
 
     // ---- phase C: per-cell register sums, two threads per cell (z halves)
     constexpr int W = Win<ORD>::W;
     const float twelfth = 1.0f / 12.0f;
     for (int wk = tid; wk < 2 * nc; wk += NT) {
-        const int c = perm[wk >> 1], half = wk & 1;
+        const int c = wk >> 1, half = wk & 1;
         const int r0 = start[c], r1 = r0 + cnt[c];
         if (r1 == r0) continue;
         const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
