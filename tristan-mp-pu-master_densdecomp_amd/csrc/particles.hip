@@ -495,14 +495,13 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     // ---- direct deposit of the y/z crossers
     const int nr = (int)(*nrec & 0xffffu);
     const int nd = (int)(*nrec >> 16);
-    // crosser k goes to wave k % nwaves, so all waves share the pass
-    const int nwaves = NT >> 6;
-    for (int k = (tid & 63) * nwaves + (tid >> 6); k < nd; k += NT) {
+    for (int k = tid; k < nd; k += NT) {
         const int r = CELLS_R - 1 - k;
         const int c = rc[r];
         const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
-        deposit<3, ORD>(snk, li + 2, lj + 2, lk + 2, rec[r], rec[CELLS_R + r], rec[2 * CELLS_R + r],
-                        rec[3 * CELLS_R + r], rec[4 * CELLS_R + r], rec[5 * CELLS_R + r], rq[r]);
+        deposit<3, ORD, LdsSink, false>(snk, li + 2, lj + 2, lk + 2, rec[r], rec[CELLS_R + r],
+                                        rec[2 * CELLS_R + r], rec[3 * CELLS_R + r], rec[4 * CELLS_R + r],
+                                        rec[5 * CELLS_R + r], rq[r]);
     }
 
     // ---- phase B: counting sort of the cell-summed records by cell

@@ -103,6 +103,33 @@ __device__ __forceinline__ int axis_deposit(float o, float dl, float* S1, float*
 {
     constexpr int W = Win<ORD>::W;
     using P = Piece<ORD>;
+    if constexpr (ORD == 3) {
+        // branch-free form of the general case below: the old piece is the
+        // cell itself (m0 = 0), the new one m1 in {-1, 0, 1}; the window
+        // starts at node -2 when m1 = -1, else at -1 (shift = 1 or 0)
+        float w[4], t0[4], t1[4];
+        P::w(o, w);
+        const int m1 = (int)floorf(o + dl);
+        const float K = (m1 > 0) ? 1.0f : 0.0f;
+        const float b0 = (m1 == 0) ? o + dl : K;
+        const float d0 = (m1 == 0) ? dl : K - o;
+        P::dw(o, b0, d0, t0);
+        const float d2 = dl - d0;              // zero without a crossing
+        const float a1 = K - (float)m1;
+        P::dw(a1, a1 + d2, d2, t1);
+        const bool sh = m1 < 0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            const float wq = sh ? (q >= 1 ? w[q - 1] : 0.0f) : (q <= 3 ? w[q] : 0.0f);
+            const float tq = sh ? (q >= 1 ? t0[q - 1] : 0.0f) : (q <= 3 ? t0[q] : 0.0f);
+            // piece m1 = +1 covers nodes 0..3 (slots 1..4 of a window from -1),
+            // m1 = -1 nodes -2..1 (slots 0..3 of a window from -2)
+            const float uq = sh ? (q <= 3 ? t1[q] : 0.0f) : (q >= 1 ? t1[q - 1] : 0.0f);
+            S1[q] = wq;
+            dS[q] = tq + uq;
+        }
+        return sh ? -2 : -1;
+    }
     const float X1 = o + dl;
     const int m0 = piece_of<ORD>(o);
     const int m1 = piece_of<ORD>(X1);

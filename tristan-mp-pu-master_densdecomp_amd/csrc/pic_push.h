@@ -156,7 +156,7 @@ __device__ __forceinline__ void gather(const Src& src, bool quirk2, int i, int j
 // (Ay Az + dSy dSz/12), A = S1 + dS/2, which is the reference's
 // S1 S1 + dS S1/2 + S1 dS/2 + dS dS/3.  The last node of each prefix sum is
 // the telescoped zero and is not written.
-template <int DIM, int ORD, class Sink>
+template <int DIM, int ORD, class Sink, bool SKIP_ZERO = true>
 __device__ __forceinline__ void deposit(const Sink& snk, int i, int j, int k, float ox, float oy, float oz,
                                         float dx, float dy, float dz, float q)
 {
@@ -199,15 +199,15 @@ __device__ __forceinline__ void deposit(const Sink& snk, int i, int j, int k, fl
                 for (int t = 0; t < W; ++t) {
                     if (t < W - 1) {
                         const float v = Cx[t] * wyz;
-                        if (v != 0.0f) snk.add(0, r, cc[t], v);
+                        if (!SKIP_ZERO || v != 0.0f) snk.add(0, r, cc[t], v);
                     }
                     if (b < W - 1) {
                         const float v = Cy[b] * (Ax[t] * Az[c] + twelfth * dSx[t] * dSz[c]);
-                        if (v != 0.0f) snk.add(1, r, cc[t], v);
+                        if (!SKIP_ZERO || v != 0.0f) snk.add(1, r, cc[t], v);
                     }
                     if (c < W - 1) {
                         const float v = Cz[c] * (Ax[t] * Ay[b] + twelfth * dSx[t] * dSy[b]);
-                        if (v != 0.0f) snk.add(2, r, cc[t], v);
+                        if (!SKIP_ZERO || v != 0.0f) snk.add(2, r, cc[t], v);
                     }
                 }
             }
@@ -222,14 +222,14 @@ __device__ __forceinline__ void deposit(const Sink& snk, int i, int j, int k, fl
             for (int t = 0; t < W; ++t) {
                 if (t < W - 1) {
                     const float v = Cx[t] * Ay[b];
-                    if (v != 0.0f) snk.add(0, r, cc[t], v);
+                    if (!SKIP_ZERO || v != 0.0f) snk.add(0, r, cc[t], v);
                 }
                 if (b < W - 1) {
                     const float v = Cy[b] * Ax[t];
-                    if (v != 0.0f) snk.add(1, r, cc[t], v);
+                    if (!SKIP_ZERO || v != 0.0f) snk.add(1, r, cc[t], v);
                 }
                 const float v = qdz * (Ax[t] * Ay[b] + twelfth * dSx[t] * dSy[b]);
-                if (v != 0.0f) snk.add(2, r, cc[t], v);
+                if (!SKIP_ZERO || v != 0.0f) snk.add(2, r, cc[t], v);
             }
         }
     }
