@@ -253,7 +253,7 @@ constexpr int CELLS_R = 2816;       // records per pass (both species)
 
 struct CellsLds {
     int fplane, jplane, nc;
-    size_t f_off, j_off, rec_off, rq_off, rc_off, idx_off, cnt_off, start_off, fill_off, bytes;
+    size_t f_off, j_off, rec_off, rq_off, rc_off, idx_off, perm_off, hist_off, cnt_off, start_off, fill_off, bytes;
 };
 
 __host__ __device__ inline CellsLds cells_lds(int FX, int FY, int FZ, int JX, int JY, int JZ, int nc)
@@ -272,6 +272,8 @@ __host__ __device__ inline CellsLds cells_lds(int FX, int FY, int FZ, int JX, in
     L.fill_off = o; o += sizeof(int) * (size_t)nc;
     L.rc_off = o;  o += sizeof(uint16_t) * (size_t)CELLS_R;
     L.idx_off = o; o += sizeof(uint16_t) * (size_t)CELLS_R;
+    L.perm_off = o; o += sizeof(uint16_t) * (size_t)((nc + 1) & ~1);
+    L.hist_off = o; o += sizeof(int) * 64;
     o = (o + 15) & ~(size_t)15;
     o += 16;       // record counter
     L.bytes = o;
@@ -372,6 +374,8 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     int* fill = reinterpret_cast<int*>(smb + L.fill_off);
     uint16_t* rc = reinterpret_cast<uint16_t*>(smb + L.rc_off);
     uint16_t* idx = reinterpret_cast<uint16_t*>(smb + L.idx_off);
+    uint16_t* perm = reinterpret_cast<uint16_t*>(smb + L.perm_off);   // cells by descending record count
+    int* hist = reinterpret_cast<int*>(smb + L.hist_off);
     // record counts packed in one word: low 16 bits cell-summed records
     // (stored from 0 up), high 16 bits y/z crossers (stored from the top down)
     unsigned* nrec = reinterpret_cast<unsigned*>(smb + L.bytes - 16);
@@ -402,6 +406,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     }
     for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
     for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
+    if (tid < 64) hist[tid] = 0;
     if (tid == 0) *nrec = 0u;
     __syncthreads();
 
@@ -504,22 +509,27 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
                                         rec[5 * CELLS_R + r], rq[r]);
     }
 
-    // ---- phase B: counting sort of the cell-summed records by cell
-    if (tid < 64) {
-        // exclusive scan of cnt[0..nc) by the first wave
+    // ---- phase B: counting sort of the cell-summed records by cell (the
+    // crossers above run on the first waves; the scans on the last two)
+    const int wv = tid >> 6, nwv = NT >> 6, lane6 = tid & 63;
+    if (wv == nwv - 1) {
+        // exclusive scan of cnt[0..nc)
         int carry = 0;
         for (int c0 = 0; c0 < nc; c0 += 64) {
-            const int c = c0 + tid;
+            const int c = c0 + lane6;
             const int v = c < nc ? cnt[c] : 0;
             int x = v;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
                 const int y = __shfl_up(x, o, 64);
-                if (tid >= o) x += y;
+                if (lane6 >= o) x += y;
             }
             if (c < nc) start[c] = carry + x - v;
             carry += __shfl(x, 63, 64);
         }
+    } else if (wv == nwv - 2) {
+        // histogram of the record counts (bins 0..63)
+        for (int c = lane6; c < nc; c += 64) atomicAdd(hist + min(cnt[c], 63), 1);
     }
     __syncthreads();
     for (int r = tid; r < nr; r += NT) {
@@ -527,13 +537,27 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         const int pos = atomicAdd(fill + c, 1);
         idx[start[c] + pos] = (uint16_t)r;
     }
+    if (wv == nwv - 2) {
+        // cells in descending record count: bin b starts after all larger
+        // bins; the cells a wave then sums together have similar loop lengths
+        const int h = hist[lane6];
+        int x = h;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_down(x, o, 64);
+            if (lane6 + o < 64) x += y;
+        }
+        hist[lane6] = x - h;
+        __builtin_amdgcn_wave_barrier();
+        for (int c = lane6; c < nc; c += 64) perm[atomicAdd(hist + min(cnt[c], 63), 1)] = (uint16_t)c;
+    }
     __syncthreads();
 
     // ---- phase C: per-cell register sums, two threads per cell (z halves)
     constexpr int W = Win<ORD>::W;
     const float twelfth = 1.0f / 12.0f;
     for (int wk = tid; wk < 2 * nc; wk += NT) {
-        const int c = wk >> 1, half = wk & 1;
+        const int c = perm[wk >> 1], half = wk & 1;
         const int r0 = start[c], r1 = r0 + cnt[c];
         if (r1 == r0) continue;
         const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
