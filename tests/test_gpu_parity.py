@@ -78,9 +78,20 @@ def test_step_vs_reference_fp64(tmpg, dim, order):
 def test_push_deposit_vs_oracle_fp64(tmpg, dim, order):
     """fused push+deposit (no filter) on a larger random state"""
     if dim == 3:
-        nx, ny, nz, npart = 16, 12, 10, 6000
+        _push_deposit_case(tmpg, dim, order, (16, 12, 10), 6000)
     else:
-        nx, ny, nz, npart = 48, 40, 1, 8000
+        _push_deposit_case(tmpg, dim, order, (48, 40, 1), 8000)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_push_deposit_dense_tile(tmpg, order):
+    """one 8x8x4 tile holding 5207 particles: more than the 3D push keeps
+    deposit records for, so the rest deposit at once"""
+    _push_deposit_case(tmpg, 3, order, (8, 8, 4), 2600)
+
+
+def _push_deposit_case(tmpg, dim, order, shape, npart):
+    nx, ny, nz = shape
     p = dict(c=0.45, corr=1.025, c_omp=4.0, gamma0=0.5, ppc0=4.0, me=1.0, mi=16.0)
     f, pi, pe = random_state(dim, order, nx, ny, nz, npart, seed=7 + 10 * dim + order)
     cfg = orc.make_cfg(dim, order, nx, ny, nz)

@@ -361,6 +361,130 @@ __device__ __forceinline__ bool fits4(float o, float d)
     return lo >= -1 && hi <= 2;
 }
 
+// Per-cell register sums of one cell's records (idx[r0 .. r0+n) index the
+// records) over the z slots of half HALF, added to the tile's LDS currents.
+template <int ORD, int HALF>
+__device__ __forceinline__ void cell_sums(int c, int r0, int n, int tx, int ty, const uint16_t* __restrict__ idx,
+                                          const float* __restrict__ rec, const float* __restrict__ rq,
+                                          const LdsSink& snk)
+{
+    const float twelfth = 1.0f / 12.0f;
+    const int r1 = r0 + n;
+    if (r1 == r0) return;
+    const int li = c % tx, lj = (c / tx) % ty, lk = c / (tx * ty);
+    // z nodes of this half: Jx, Jy use z window slots {0,1} or {2,3};
+    // Jz prefix slots {0,1} or {2}
+    constexpr int zb = HALF ? 2 : 0;
+    float ax[5][4][2], ay[6][3][2], az[6][4][2];
+#pragma unroll
+    for (int u = 0; u < 5; ++u)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) { ax[u][b][0] = 0.0f; ax[u][b][1] = 0.0f; }
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) { ay[u][b][0] = 0.0f; ay[u][b][1] = 0.0f; }
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) { az[u][b][0] = 0.0f; az[u][b][1] = 0.0f; }
+    for (int rr = r0; rr < r1; ++rr) {
+        const int r = idx[rr];
+        const float ox = rec[r], oy = rec[CELLS_R + r], oz = rec[2 * CELLS_R + r];
+        const float dx = rec[3 * CELLS_R + r], dy = rec[4 * CELLS_R + r], dz = rec[5 * CELLS_R + r];
+        const float q = rq[r];
+        float dSx[6], Ax[6], dSy[4], Ay[4], dSz[4], Az[4];
+        {
+            float S6[6];
+            profile6<ORD>(ox, dx, S6, dSx);
+#pragma unroll
+            for (int u = 0; u < 6; ++u) Ax[u] = S6[u] + 0.5f * dSx[u];
+        }
+        {
+            float S4[4];
+            profile4<ORD>(oy, dy, S4, dSy);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Ay[u] = S4[u] + 0.5f * dSy[u];
+        }
+        {
+            float S4[4];
+            profile4<ORD>(oz, dz, S4, dSz);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Az[u] = S4[u] + 0.5f * dSz[u];
+        }
+        float Cx[5], Cy[3], Cz[3];
+        {
+            float run = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 5; ++u) { run += dSx[u]; Cx[u] = q * run; }
+            run = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 3; ++u) { run += dSy[u]; Cy[u] = q * run; }
+            run = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 3; ++u) { run += dSz[u]; Cz[u] = q * run; }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int zc = zb + h;       // z window slot 0..3
+            const float azc = HALF ? Az[2 + h] : Az[h];
+            const float dzc = HALF ? dSz[2 + h] : dSz[h];
+            (void)zc;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const float wyz = Ay[b] * azc + twelfth * dSy[b] * dzc;
+#pragma unroll
+                for (int u = 0; u < 5; ++u) ax[u][b][h] += Cx[u] * wyz;
+            }
+#pragma unroll
+            for (int u = 0; u < 6; ++u) {
+                const float wxz = Ax[u] * azc + twelfth * dSx[u] * dzc;
+#pragma unroll
+                for (int b = 0; b < 3; ++b) ay[u][b][h] += Cy[b] * wxz;
+            }
+            // Jz prefix slot: HALF 0 -> {0,1}, HALF 1 -> {2} only
+            if (HALF == 0 || h == 0) {
+                const float czc = HALF ? Cz[2] : Cz[h];
+#pragma unroll
+                for (int u = 0; u < 6; ++u)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        az[u][b][h] += czc * (Ax[u] * Ay[b] + twelfth * dSx[u] * dSy[b]);
+            }
+        }
+    }
+    // add the cell's sums to the tile currents: node (li-2+u, lj-1+b,
+    // lk-1+zslot) sits at LDS (li+u, lj+1+b, lk+1+zslot)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int zs = zb + h;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int row = snk.row(lj + 1 + b, lk + 1 + zs);
+#pragma unroll
+            for (int u = 0; u < 5; ++u)
+                snk.add(0, row, li + u, ax[u][b][h]);
+        }
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int row = snk.row(lj + 1 + b, lk + 1 + zs);
+#pragma unroll
+            for (int u = 0; u < 6; ++u)
+                snk.add(1, row, li + u, ay[u][b][h]);
+        }
+        if (HALF == 0 || h == 0) {
+            const int zsz = HALF ? 2 : h;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int row = snk.row(lj + 1 + b, lk + 1 + zsz);
+#pragma unroll
+                for (int u = 0; u < 6; ++u)
+                    snk.add(2, row, li + u, az[u][b][h]);
+            }
+        }
+    }
+}
+
 template <int ORD, bool HAS_CH, bool HAS_ID>
 __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
 {
@@ -553,125 +677,12 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     }
     __syncthreads();
 
-    // ---- phase C: per-cell register sums, two threads per cell (z halves)
-    constexpr int W = Win<ORD>::W;
-    const float twelfth = 1.0f / 12.0f;
+    // ---- phase C: per-cell register sums, two threads per cell (z halves);
+    // the half is uniform per wave, so each half is its own code path
     for (int wk = tid; wk < 2 * nc; wk += NT) {
-        const int c = perm[wk >> 1], half = wk & 1;
-        const int r0 = start[c], r1 = r0 + cnt[c];
-        if (r1 == r0) continue;
-        const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
-        // z nodes of this half: Jx, Jy use z window slots {0,1} or {2,3};
-        // Jz prefix slots {0,1} or {2}
-        const int zb = half ? 2 : 0;
-        float ax[5][4][2], ay[6][3][2], az[6][4][2];
-#pragma unroll
-        for (int u = 0; u < 5; ++u)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) { ax[u][b][0] = 0.0f; ax[u][b][1] = 0.0f; }
-#pragma unroll
-        for (int u = 0; u < 6; ++u)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) { ay[u][b][0] = 0.0f; ay[u][b][1] = 0.0f; }
-#pragma unroll
-        for (int u = 0; u < 6; ++u)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) { az[u][b][0] = 0.0f; az[u][b][1] = 0.0f; }
-        for (int rr = r0; rr < r1; ++rr) {
-            const int r = idx[rr];
-            const float ox = rec[r], oy = rec[CELLS_R + r], oz = rec[2 * CELLS_R + r];
-            const float dx = rec[3 * CELLS_R + r], dy = rec[4 * CELLS_R + r], dz = rec[5 * CELLS_R + r];
-            const float q = rq[r];
-            float dSx[6], Ax[6], dSy[4], Ay[4], dSz[4], Az[4];
-            {
-                float S6[6];
-                profile6<ORD>(ox, dx, S6, dSx);
-#pragma unroll
-                for (int u = 0; u < 6; ++u) Ax[u] = S6[u] + 0.5f * dSx[u];
-            }
-            {
-                float S4[4];
-                profile4<ORD>(oy, dy, S4, dSy);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) Ay[u] = S4[u] + 0.5f * dSy[u];
-            }
-            {
-                float S4[4];
-                profile4<ORD>(oz, dz, S4, dSz);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) Az[u] = S4[u] + 0.5f * dSz[u];
-            }
-            float Cx[5], Cy[3], Cz[3];
-            {
-                float run = 0.0f;
-#pragma unroll
-                for (int u = 0; u < 5; ++u) { run += dSx[u]; Cx[u] = q * run; }
-                run = 0.0f;
-#pragma unroll
-                for (int u = 0; u < 3; ++u) { run += dSy[u]; Cy[u] = q * run; }
-                run = 0.0f;
-#pragma unroll
-                for (int u = 0; u < 3; ++u) { run += dSz[u]; Cz[u] = q * run; }
-            }
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int zc = zb + h;       // z window slot 0..3
-                const float azc = half ? Az[2 + h] : Az[h];
-                const float dzc = half ? dSz[2 + h] : dSz[h];
-                (void)zc;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const float wyz = Ay[b] * azc + twelfth * dSy[b] * dzc;
-#pragma unroll
-                    for (int u = 0; u < 5; ++u) ax[u][b][h] += Cx[u] * wyz;
-                }
-#pragma unroll
-                for (int u = 0; u < 6; ++u) {
-                    const float wxz = Ax[u] * azc + twelfth * dSx[u] * dzc;
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) ay[u][b][h] += Cy[b] * wxz;
-                }
-                // Jz prefix slot: half 0 -> {0,1}, half 1 -> {2} only
-                if (half == 0 || h == 0) {
-                    const float czc = half ? Cz[2] : Cz[h];
-#pragma unroll
-                    for (int u = 0; u < 6; ++u)
-#pragma unroll
-                        for (int b = 0; b < 4; ++b)
-                            az[u][b][h] += czc * (Ax[u] * Ay[b] + twelfth * dSx[u] * dSy[b]);
-                }
-            }
-        }
-        // add the cell's sums to the tile currents: node (li-2+u, lj-1+b,
-        // lk-1+zslot) sits at LDS (li+u, lj+1+b, lk+1+zslot)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int zs = zb + h;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int row = snk.row(lj + 1 + b, lk + 1 + zs);
-#pragma unroll
-                for (int u = 0; u < 5; ++u)
-                    snk.add(0, row, li + u, ax[u][b][h]);
-            }
-#pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                const int row = snk.row(lj + 1 + b, lk + 1 + zs);
-#pragma unroll
-                for (int u = 0; u < 6; ++u)
-                    snk.add(1, row, li + u, ay[u][b][h]);
-            }
-            if (half == 0 || h == 0) {
-                const int zsz = half ? 2 : h;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int row = snk.row(lj + 1 + b, lk + 1 + zsz);
-#pragma unroll
-                    for (int u = 0; u < 6; ++u)
-                        snk.add(2, row, li + u, az[u][b][h]);
-                }
-            }
-        }
+        const int c = perm[wk % nc];
+        if (wk < nc) cell_sums<ORD, 0>(c, start[c], cnt[c], tl.tx, tl.ty, idx, rec, rq, snk);
+        else cell_sums<ORD, 1>(c, start[c], cnt[c], tl.tx, tl.ty, idx, rec, rq, snk);
     }
     __syncthreads();
     for (int e = tid; e < jplane; e += NT) {

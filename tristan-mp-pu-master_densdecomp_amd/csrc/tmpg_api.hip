@@ -552,7 +552,12 @@ int tmpg_last_timings(tmpg_ctx* ctx, double ms[6])
         (void)hipEventElapsedTime(&t[2], ev[2], ev[3]);
         (void)hipEventElapsedTime(&t[3], ev[3], ev[4]);
         (void)hipEventElapsedTime(&t[4], ev[4], ev[5]);
-        (void)hipEventElapsedTime(&t[5], ev[6], ev[7]);
+        // the push kernel's events are recorded only when it ran (no
+        // particles: none); a failed query leaves no sticky error behind
+        if (hipEventElapsedTime(&t[5], ev[6], ev[7]) != hipSuccess) {
+            t[5] = 0.0f;
+            (void)hipGetLastError();
+        }
         ms[0] += (double)t[0] + t[2] + t[4];
         ms[1] += t[1];
         ms[2] += t[3];
