@@ -17,6 +17,12 @@
 !                2 deposit only (reset, deposit, fold), 3 filter only
 !                (state-in currents are read after the fields),
 !                4 Yee only (B half, B half, E), 5 deposit + fold + filter
+!   optional   : <history-file> <every>: in mode 0, every <every> steps one
+!                line "step E_energy B_energy KE_ions KE_electrons" in FP64:
+!                0.5*sum(E^2), 0.5*sum(B^2) over the interior cells and
+!                sum((gamma-1) c^2 m) per species, the quantities of the
+!                reference's energy() (code/tristanmainloop.F90:384-494)
+!                without its index-range and normalisation choices
 !
 ! The main loop itself (code/tristanmainloop.F90) cannot be compiled here:
 ! it depends on m_output, whose save_param calls HDF5 outside #ifdef HDF5
@@ -55,6 +61,8 @@ program tristan_ref
 	integer(kind=4) :: ni4, ne4
 	real(sprec), allocatable :: buf(:)
 	real(sprec) :: lc
+	character(len=512) :: histfile
+	integer :: every
 
 	call get_command_argument(1, inpfile)
 	call get_command_argument(2, fin)
@@ -63,6 +71,13 @@ program tristan_ref
 	read(arg,*) nsteps
 	call get_command_argument(5, arg)
 	read(arg,*) mode
+	histfile = ""
+	every = 0
+	if (command_argument_count() .ge. 7) then
+		call get_command_argument(6, histfile)
+		call get_command_argument(7, arg)
+		read(arg,*) every
+	endif
 
 	call inputpar_open(trim(inpfile))
 	call read_input_communications()
@@ -109,6 +124,7 @@ program tristan_ref
 	call read_species(maxhlf + 1, ne)
 	close(21)
 
+	if (every .gt. 0) open(unit=23, file=trim(histfile), status='replace', action='write')
 	do step = 1, nsteps
 		select case (mode)
 		case (0)
@@ -167,7 +183,11 @@ program tristan_ref
 			call exchange_particles()
 			call inject_others()
 		end select
+		if (every .gt. 0) then
+			if (mod(step, every) .eq. 0) call write_energies(step)
+		endif
 	enddo
+	if (every .gt. 0) close(23)
 
 	open(unit=22, file=trim(fout), access='stream', form='unformatted', status='replace')
 	write(22) ex, ey, ez, bx, by, bz, curx, cury, curz
@@ -182,6 +202,45 @@ program tristan_ref
 	call MPI_Finalize(ierr2)
 
 contains
+
+	subroutine write_energies(st)
+		integer, intent(in) :: st
+		integer :: i, j, k, i1, i2, j1, j2, k1, k2
+		real(8) :: ee, be, ki, ke
+		i1 = nghost / 2 + 1
+		i2 = mx0 - (nghost - nghost / 2)
+		j1 = i1
+		j2 = my0 - (nghost - nghost / 2)
+#ifdef twoD
+		k1 = 1
+		k2 = 1
+#else
+		k1 = nghostz / 2 + 1
+		k2 = mz0 - (nghostz - nghostz / 2)
+#endif
+		ee = 0d0
+		be = 0d0
+		do k = k1, k2
+			do j = j1, j2
+				do i = i1, i2
+					ee = ee + real(ex(i,j,k), 8)**2 + real(ey(i,j,k), 8)**2 + real(ez(i,j,k), 8)**2
+					be = be + real(bx(i,j,k), 8)**2 + real(by(i,j,k), 8)**2 + real(bz(i,j,k), 8)**2
+				enddo
+			enddo
+		enddo
+		ki = 0d0
+		do n = 1, ions
+			ki = ki + (sqrt(1d0 + real(p(n)%u, 8)**2 + real(p(n)%v, 8)**2 + real(p(n)%w, 8)**2) - 1d0) &
+				* real(p(n)%ch, 8)
+		enddo
+		ke = 0d0
+		do n = maxhlf + 1, maxhlf + lecs
+			ke = ke + (sqrt(1d0 + real(p(n)%u, 8)**2 + real(p(n)%v, 8)**2 + real(p(n)%w, 8)**2) - 1d0) &
+				* real(p(n)%ch, 8)
+		enddo
+		write(23, '(i8, 4es25.16)') st, 0.5d0 * ee, 0.5d0 * be, ki * real(c, 8)**2 * real(mi, 8), &
+			ke * real(c, 8)**2 * real(me, 8)
+	end subroutine write_energies
 
 	subroutine read_species(i0, cnt)
 		integer, intent(in) :: i0, cnt

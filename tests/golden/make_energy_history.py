@@ -1,0 +1,133 @@
+"""Energy histories of the compiled reference over many main-loop steps.
+
+A counter-streaming pair-plasma (Weibel) setup is run by the reference's own
+routines (oracle/_ref/tristan_ref_*, FP32 and FP64 builds), writing every
+EVERY steps 0.5*sum(E^2), 0.5*sum(B^2) and the kinetic energy of each
+species.  The initial state is regenerated from its seed by `initial_state`
+(numpy's PCG64 stream), so the fixture stores only the parameters and the
+histories; the GPU test rebuilds the same state and compares its own history
+with both.
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+import orc  # noqa: E402
+from make_golden import REF_DIR, write_state  # noqa: E402
+
+CASES = {
+    # 2D: 64 x 48 cells, 8 particles per cell per species, 2nd order
+    "weibel2d_ord2": dict(dim=2, order=2, nx=64, ny=48, nz=1, ppc=8, steps=1000, seed=71),
+    # 3D: 16^3 cells, 4 particles per cell per species, 3rd order
+    "weibel3d_ord3": dict(dim=3, order=3, nx=16, ny=16, nz=16, ppc=4, steps=500, seed=72),
+}
+PHYS = dict(c=0.45, corr=1.025, ntimes=4, c_omp=5.0, gamma0=0.5, me=1.0, mi=1.0, gb=0.5, theta=2.5e-3)
+EVERY = 10
+
+
+def input_text(case, maxptl):
+    p = PHYS
+    return f"""<node_configuration>
+sizex = 1
+sizey = 1
+<time>
+last = {case['steps']}
+c = {p['c']}
+<grid>
+mx0 = {case['nx']}
+my0 = {case['ny']}
+mz0 = {case['nz']}
+<algorithm>
+highorder = 0
+Corr = {p['corr']}
+ntimes = {p['ntimes']}
+<boundaries>
+periodicx = 1
+periodicy = 1
+periodicz = 1
+<fields>
+btheta = 0
+bphi = 90
+<domain>
+enlarge = 0
+<particles>
+sigma = 0.
+maxptl0 = {maxptl}
+ppc0 = {2 * case['ppc']}
+delgam = {p['theta']}
+me = {p['me']}
+mi = {p['mi']}
+gamma0 = {p['gamma0']}
+c_omp = {p['c_omp']}
+<problem>
+distr_dim = 3
+temperature_ratio = 1
+external_fields = 0
+sigma_ext = 0.
+user_part_bcs = 0
+"""
+
+
+def physics(case):
+    """normalisation of the reference (code/particles.F90:206-235), ppc0 = 2 ppc"""
+    p = dict(PHYS)
+    p["ppc0"] = 2.0 * case["ppc"]
+    return p
+
+
+def initial_state(case):
+    """uniform positions; half of each species drifts at -gb, half at +gb
+    along x, with a Gaussian thermal spread sqrt(theta) per component"""
+    rng = np.random.default_rng(case["seed"])
+    cfg = orc.make_cfg(case["dim"], case["order"], case["nx"], case["ny"], case["nz"])
+    mx, my, mz, ng, ngz = orc.extent(cfg)
+    h, hz = ng // 2, ngz // 2
+    ncell = case["nx"] * case["ny"] * case["nz"]
+    n = ncell * case["ppc"]
+    out = []
+    for _ in range(2):
+        pos = np.empty((3, n))
+        pos[0] = h + 1 + rng.random(n) * case["nx"]
+        pos[1] = h + 1 + rng.random(n) * case["ny"]
+        pos[2] = (hz + 1 + rng.random(n) * case["nz"]) if case["dim"] == 3 else hz + 1.5
+        mom = rng.standard_normal((3, n)) * np.sqrt(PHYS["theta"])
+        mom[0, : n // 2] -= PHYS["gb"]
+        mom[0, n // 2:] += PHYS["gb"]
+        out.append(np.concatenate([pos, mom, np.ones((1, n))]).astype(np.float32))
+    return cfg, out[0], out[1]
+
+
+def run_history(case, prec):
+    import subprocess
+    cfg, pi, pe = initial_state(case)
+    dt = np.float32 if prec == "f32" else np.float64
+    exe = os.path.join(REF_DIR, f"tristan_ref_{case['dim']}d_dd{case['order']}_{prec}")
+    maxptl = 2 * (pi.shape[1] + pe.shape[1]) + 1000
+    mx, my, mz, _, _ = orc.extent(cfg)
+    with tempfile.TemporaryDirectory() as td:
+        open(os.path.join(td, "input"), "w").write(input_text(case, maxptl))
+        st = os.path.join(td, "state.bin")
+        write_state(st, np.zeros((6, mz, my, mx), dt), pi, pe, dt)
+        hist = os.path.join(td, "hist.txt")
+        subprocess.run([exe, "input", st, os.path.join(td, "out.bin"), str(case["steps"]), "0", hist, str(EVERY)],
+                       check=True, cwd=td, stdout=subprocess.DEVNULL)
+        return np.loadtxt(hist)
+
+
+def main():
+    for name, case in CASES.items():
+        fix = dict(params=np.array(repr(dict(case, every=EVERY, **physics(case)))))
+        for prec in ("f32", "f64"):
+            fix[f"hist_{prec}"] = run_history(case, prec)
+        out = os.path.join(HERE, f"energy_{name}.npz")
+        np.savez_compressed(out, **fix)
+        print(out, os.path.getsize(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,80 @@
+"""Field and particle energy histories of many main-loop steps against the
+compiled reference (tests/golden/energy_*.npz, made by
+tests/golden/make_energy_history.py from the reference's FP32 and FP64
+builds).
+
+The same seeded counter-streaming pair plasma (Weibel) runs in the library
+for 1000 (2D, order 2) or 500 (3D, order 3) steps.  Round-off differences
+grow in a many-particle system, so the bar is set by the reference against
+itself: the library's deviation from the FP64 reference may be at most a
+small multiple of the reference FP32 build's own deviation from it (plus a
+floor of FP32 round-off), and tighter while the instability still grows
+from the identical seed noise.
+"""
+import ast
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["E", "B", "KE_ions", "KE_electrons", "total"]
+
+
+def _gpu_history(tmpg, case, p, pi, pe):
+    from gpu_util import make_context
+    import orc
+    dim, order = case["dim"], case["order"]
+    ctx = make_context(tmpg, dim, order, case["nx"], case["ny"], case["nz"], p, ntimes=p["ntimes"],
+                       track_ids=False)
+    cfg = orc.make_cfg(dim, order, case["nx"], case["ny"], case["nz"])
+    mx, my, mz, _, _ = orc.extent(cfg)
+    ctx.set_fields(np.zeros((6, mz, my, mx), np.float32))
+    ctx.set_particles(0, pi)
+    ctx.set_particles(1, pe)
+    every = case["every"]
+    rows = []
+    for k in range(case["steps"] // every):
+        ctx.step(every)
+        d = ctx.diag()
+        rows.append([(k + 1) * every, d[0], d[1], d[2], d[3]])
+    ctx.close()
+    return np.array(rows)
+
+
+def _with_total(h):
+    return np.concatenate([h[:, 1:5], h[:, 1:5].sum(1, keepdims=True)], axis=1)
+
+
+@pytest.mark.parametrize("name", ["weibel2d_ord2", "weibel3d_ord3"])
+def test_energy_history_tracks_reference(name):
+    import tmpg
+    import make_energy_history as meh
+    fx = np.load(os.path.join(HERE, "golden", f"energy_{name}.npz"))
+    case = ast.literal_eval(str(fx["params"]))
+    _, pi, pe = meh.initial_state(case)
+    h32, h64 = fx["hist_f32"], fx["hist_f64"]
+    hg = _gpu_history(tmpg, case, case, pi, pe)
+    assert np.array_equal(hg[:, 0], h64[:, 0])
+    g, r32, r64 = _with_total(hg), _with_total(h32), _with_total(h64)
+    dg = np.abs(g - r64) / np.abs(r64)
+    d32 = np.abs(r32 - r64) / np.abs(r64)
+    n = len(r64)
+    for q, qn in enumerate(NAMES):
+        print(f"{name} {qn:12s} max dev GPU {dg[:, q].max():.2e} (first half {dg[: n // 2, q].max():.2e}), "
+              f"reference FP32 {d32[:, q].max():.2e} (first half {d32[: n // 2, q].max():.2e})")
+    # energies of the whole run: within 3x the reference FP32 build's own
+    # deviation from its FP64 build, or 1e-4 (2e-5 for the conserved total)
+    for q, qn in enumerate(NAMES):
+        floor = 2e-5 if qn == "total" else 1e-4
+        assert dg[:, q].max() <= max(3.0 * d32[:, q].max(), floor), (qn, dg[:, q].max(), d32[:, q].max())
+    # first half (linear growth from the identical seed noise): within 2x
+    # the reference FP32 build's deviation, or 2e-5
+    for q, qn in enumerate(NAMES):
+        a, b = dg[: n // 2, q].max(), d32[: n // 2, q].max()
+        assert a <= max(2.0 * b, 2e-5), (qn, a, b)
