@@ -170,8 +170,13 @@ int tmpg_model_set_params(tmpg_model* m, const float* in);
 int tmpg_model_set_scale(tmpg_model* m, float s);
 int tmpg_model_get_scale(const tmpg_model* m, float* s);
 int tmpg_model_set_base_passes(tmpg_model* m, int npasses);
+/* training loss = mean(d^2) + lam * <d,t>^2 / (N <t,t>) per image, d = output
+ * - target, t = target (both / s): lam > 0 also penalises the error along the
+ * target (a shrunk or inflated amplitude); 0 (default) is plain MSE */
+int tmpg_model_set_amplitude_weight(tmpg_model* m, float lam);
 int tmpg_model_get_base_passes(const tmpg_model* m, int* npasses);
-/* one forward/backward on a batch: loss = mean((J_lo/s + curl A - J_t/s)^2);
+/* one forward/backward on a batch: loss = mean((J_lo/s + curl A - J_t/s)^2)
+ * (+ the amplitude term);
  * grad_out (may be NULL) receives the flat gradient; update != 0 applies Adam */
 int tmpg_model_train_batch(tmpg_model* m, int nx, int ny, int nz, int batch, const float* jlo,
                            const float* jt, float lr, int update, double* loss, float* grad_out);

@@ -134,8 +134,11 @@ class Net:
             out += unshift(dz @ Wr[:, t, :], d)
         return out
 
-    def loss_and_grad(self, jlo, jt, s, bf=True, grad=True):
-        """jlo, jt: (3, nz, ny, nx) current arrays; returns loss, flat grad"""
+    def loss_and_grad(self, jlo, jt, s, bf=True, grad=True, lam=0.0):
+        """jlo, jt: (3, nz, ny, nx) current arrays; returns loss, flat grad.
+        loss = mean(d^2) + lam <d,t>^2 / (N <t,t>), d = prediction - target,
+        t = target (both / s): the second term penalises the error along the
+        target, i.e. a shrunk or inflated amplitude."""
         r = bf16 if bf else _id
         f32 = (lambda a: np.asarray(a, np.float32).astype(np.float64)) if bf else _id
         inv_s = np.float32(1.0) / np.float32(s) if bf else 1.0 / s
@@ -152,10 +155,13 @@ class Net:
         c = curl_b(A, self.dim)
         d = xl + c - xt
         N = d.size
-        loss = float(np.sum(d * d) / N)
+        tt = float(np.sum(xt * xt))
+        dt = float(np.sum(d * xt))
+        bias = dt / tt if (lam and tt > 0.0) else 0.0
+        loss = float(np.sum(d * d) / N) + lam * bias * dt / N
         if not grad:
             return loss, None, xl + c
-        g = 2.0 / N * d
+        g = 2.0 / N * (d + lam * bias * xt)
         dz = r(curl_b_adj(g, self.dim))
         grads = [None] * self.nl
         for l in range(self.nl - 1, -1, -1):

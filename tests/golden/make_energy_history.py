@@ -25,6 +25,9 @@ CASES = {
     "weibel2d_ord2": dict(dim=2, order=2, nx=64, ny=48, nz=1, ppc=8, steps=1000, seed=71),
     # 3D: 16^3 cells, 4 particles per cell per species, 3rd order
     "weibel3d_ord3": dict(dim=3, order=3, nx=16, ny=16, nz=16, ppc=4, steps=500, seed=72),
+    # 2D: 256^2 cells, 8 particles per cell per species, 1st order, c/wp = 10
+    # (the size of the reference's CPU-runnable 2D configuration)
+    "weibel2d_ord1_256": dict(dim=2, order=1, nx=256, ny=256, nz=1, ppc=8, steps=1000, seed=73, c_omp=10.0),
 }
 PHYS = dict(c=0.45, corr=1.025, ntimes=4, c_omp=5.0, gamma0=0.5, me=1.0, mi=1.0, gb=0.5, theta=2.5e-3)
 EVERY = 10
@@ -63,7 +66,7 @@ delgam = {p['theta']}
 me = {p['me']}
 mi = {p['mi']}
 gamma0 = {p['gamma0']}
-c_omp = {p['c_omp']}
+c_omp = {case.get('c_omp', p['c_omp'])}
 <problem>
 distr_dim = 3
 temperature_ratio = 1
@@ -77,6 +80,7 @@ def physics(case):
     """normalisation of the reference (code/particles.F90:206-235), ppc0 = 2 ppc"""
     p = dict(PHYS)
     p["ppc0"] = 2.0 * case["ppc"]
+    p["c_omp"] = case.get("c_omp", PHYS["c_omp"])
     return p
 
 
@@ -119,9 +123,11 @@ def run_history(case, prec):
         return np.loadtxt(hist)
 
 
-def main():
+def main(names=None):
     for name, case in CASES.items():
-        fix = dict(params=np.array(repr(dict(case, every=EVERY, **physics(case)))))
+        if names and name not in names:
+            continue
+        fix = dict(params=np.array(repr({**case, **physics(case), "every": EVERY})))
         for prec in ("f32", "f64"):
             fix[f"hist_{prec}"] = run_history(case, prec)
         out = os.path.join(HERE, f"energy_{name}.npz")
@@ -130,4 +136,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

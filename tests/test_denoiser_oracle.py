@@ -39,8 +39,8 @@ def test_curl_matches_yee_e_update_stencil():
     assert c[1, 1, 1, 1] == -1.0 and c[1, 1, 2, 1] == 1.0     # -d/dx Az
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_oracle_gradient_matches_finite_differences(dim):
+@pytest.mark.parametrize("dim,lam", [(2, 0.0), (3, 0.0), (2, 2.0), (3, 2.0)])
+def test_oracle_gradient_matches_finite_differences(dim, lam):
     rng = np.random.default_rng(10 + dim)
     shape = (3, 4, 5) if dim == 3 else (1, 6, 5)
     net = dr.Net(dim, width=4, nlayers=3)
@@ -48,14 +48,14 @@ def test_oracle_gradient_matches_finite_differences(dim):
     net.set_flat(p)
     jlo = rng.standard_normal((3,) + shape)
     jt = rng.standard_normal((3,) + shape)
-    loss, g, _ = net.loss_and_grad(jlo, jt, 1.3, bf=False)
+    loss, g, _ = net.loss_and_grad(jlo, jt, 1.3, bf=False, lam=lam)
     idx = rng.choice(p.size, 25, replace=False)
     eps = 1e-6
     for k in idx:
         q = p.copy(); q[k] += eps; net.set_flat(q)
-        lp, _, _ = net.loss_and_grad(jlo, jt, 1.3, bf=False, grad=False)
+        lp, _, _ = net.loss_and_grad(jlo, jt, 1.3, bf=False, grad=False, lam=lam)
         q[k] -= 2 * eps; net.set_flat(q)
-        lm, _, _ = net.loss_and_grad(jlo, jt, 1.3, bf=False, grad=False)
+        lm, _, _ = net.loss_and_grad(jlo, jt, 1.3, bf=False, grad=False, lam=lam)
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - g[k]) < 1e-6 * max(1.0, abs(fd)), (k, fd, g[k])
 
@@ -66,3 +66,17 @@ def test_bf16_rounding():
     assert r[0] == 1.0 and r[1] == 1.0          # tie to even
     assert r[2] == 1.0078125                      # rounds up
     assert r[4] == -2.5
+
+
+def test_amplitude_term_is_the_projection_of_the_error():
+    """lam <d,t>^2/(N<t,t>) is lam * N * mean(d_par^2) for d_par the part of
+    the error along the target: a prediction that is the target scaled by
+    (1 + e) costs (1 + lam) e^2 <t,t>/N"""
+    rng = np.random.default_rng(5)
+    net = dr.Net(2, width=4, nlayers=2)
+    net.set_flat(np.zeros(net.nparams()))      # A = 0: prediction = input
+    jt = rng.standard_normal((3, 1, 6, 5))
+    e = 0.1
+    for lam in (0.0, 3.0):
+        loss, _, _ = net.loss_and_grad((1 + e) * jt, jt, 1.0, bf=False, grad=False, lam=lam)
+        assert abs(loss - (1 + lam) * e * e * np.sum(jt * jt) / jt.size) < 1e-12

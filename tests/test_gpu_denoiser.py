@@ -34,8 +34,9 @@ def smooth_pair(rng, dim, shape):
     return jlo, jt
 
 
-@pytest.mark.parametrize("dim,shape,batch", [(2, (1, 24, 32), 2), (3, (8, 10, 12), 1)])
-def test_loss_and_gradients_match_oracle(tmpg, dim, shape, batch):
+@pytest.mark.parametrize("dim,shape,batch,lam", [(2, (1, 24, 32), 2, 0.0), (3, (8, 10, 12), 1, 0.0),
+                                                 (2, (1, 24, 32), 2, 3.0), (3, (8, 10, 12), 1, 3.0)])
+def test_loss_and_gradients_match_oracle(tmpg, dim, shape, batch, lam):
     rng = np.random.default_rng(dim)
     m = tmpg.Model(dim, seed=5)
     p = m.get_params().astype(np.float64)
@@ -48,11 +49,12 @@ def test_loss_and_gradients_match_oracle(tmpg, dim, shape, batch):
     jt = np.stack([b for _, b in pairs])
     s = 0.9
     m.set_scale(s)
+    m.set_amplitude_weight(lam)
     loss, g = m.train_batch(jlo, jt)
     net = dr.Net(dim, params=p32)
     lo, go = 0.0, np.zeros_like(g, dtype=np.float64)
     for b in range(batch):
-        l_, g_, _ = net.loss_and_grad(jlo[b], jt[b], s, bf=True)
+        l_, g_, _ = net.loss_and_grad(jlo[b], jt[b], s, bf=True, lam=lam)
         lo += l_ / batch
         go += g_ / batch
     assert abs(loss - lo) < 1e-4 * lo, (loss, lo)

@@ -290,22 +290,64 @@ __global__ void k_curl_adj(Geo g, const float* __restrict__ gc, float* __restric
     dA[p * 3 + 2] = az;
 }
 
-// pred = x + curl(A); loss += sum (pred - t)^2 / N; g = 2 (pred - t) / N.
+// per image b: dots[2b] = <d, t>, dots[2b+1] = <t, t> with d = pred - t
+// (FP64 sums); a wave's pixels all belong to one image when n % 64 == 0,
+// else each lane adds its own terms
+__global__ void k_dots(int64_t n, int batch, const float* __restrict__ jlo, const float* __restrict__ jt,
+                       float inv_s, const float* __restrict__ curl, double* dots)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double dt = 0.0, tt = 0.0;
+    int64_t b = 0;
+    if (t < n * batch) {
+        b = t / n;
+        const int64_t p = t - b * n;
+        for (int c = 0; c < 3; ++c) {
+            const float pred = jlo[(b * 3 + c) * n + p] * inv_s + curl[t * 3 + c];
+            const float tv = jt[(b * 3 + c) * n + p] * inv_s;
+            const float d = pred - tv;
+            dt += (double)d * tv;
+            tt += (double)tv * tv;
+        }
+    }
+    if (n % 64 == 0) {
+        for (int o = 32; o > 0; o >>= 1) {
+            dt += __shfl_down(dt, o, 64);
+            tt += __shfl_down(tt, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0 && t < n * batch) {
+            atomicAdd(dots + 2 * b, dt);
+            atomicAdd(dots + 2 * b + 1, tt);
+        }
+    } else if (t < n * batch) {
+        atomicAdd(dots + 2 * b, dt);
+        atomicAdd(dots + 2 * b + 1, tt);
+    }
+}
+
+// pred = x + curl(A), d = pred - t; loss += sum d^2 / N, g = 2 d / N; with
+// the amplitude term (lam > 0) the image's loss gains lam <d,t>^2/(N <t,t>)
+// and g gains 2 lam (<d,t>/<t,t>) t / N.
 // x, t FP32 [batch][3][n] (the FP32 currents / s); curl [P][3]
 __global__ void k_loss(int64_t n, int batch, const float* __restrict__ jlo, const float* __restrict__ jt,
                        float inv_s, const float* __restrict__ curl, float* __restrict__ grad, double* loss,
-                       double invN)
+                       double invN, float lam, const double* __restrict__ dots)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;
     if (t < n * batch) {
         const int64_t b = t / n, p = t - b * n;
+        float bias = 0.0f;
+        if (lam > 0.0f && dots[2 * b + 1] > 0.0) bias = (float)((double)lam * dots[2 * b] / dots[2 * b + 1]);
         for (int c = 0; c < 3; ++c) {
             const float pred = jlo[(b * 3 + c) * n + p] * inv_s + curl[t * 3 + c];
-            const float d = pred - jt[(b * 3 + c) * n + p] * inv_s;
+            const float tv = jt[(b * 3 + c) * n + p] * inv_s;
+            const float d = pred - tv;
             s += (double)d * d;
-            if (grad) grad[t * 3 + c] = (float)(2.0 * invN) * d;
+            if (grad) grad[t * 3 + c] = (float)(2.0 * invN) * (d + bias * tv);
         }
+        if (p == 0 && lam > 0.0f && dots[2 * b + 1] > 0.0)
+            s += (double)lam * dots[2 * b] * dots[2 * b] / dots[2 * b + 1];
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(loss, s * invN);
@@ -417,6 +459,8 @@ struct tmpg_model {
     uint16_t* dz = nullptr;
     float *jlo = nullptr, *jt = nullptr;
     double* dloss = nullptr;
+    double* dots = nullptr;              // per image <d,t>, <t,t> (amplitude term)
+    float lam = 0.0f;                    // weight of the amplitude term
     // device-resident training set
     std::vector<float*> set_lo, set_t;
     int set_nx = 0, set_ny = 0, set_nz = 0;
@@ -425,6 +469,7 @@ struct tmpg_model {
 namespace {
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+constexpr int kMaxBatch = 64;        // images per training batch with the amplitude term
 
 #define DN_CHECK(call)                                                                 \
     do {                                                                               \
@@ -567,8 +612,15 @@ int run_batch(tmpg_model* m, const Geo& g, const float* jlo, const float* jt, bo
                        m->x0);
     forward(m, g);
     DN_CHECK(hipMemsetAsync(m->dloss, 0, sizeof(double), m->stream));
+    if (m->lam > 0.0f) {
+        if (g.batch > kMaxBatch) { m->err = "batch too large for the amplitude term"; return TMPG_EINVAL; }
+        DN_CHECK(hipMemsetAsync(m->dots, 0, 2 * (size_t)g.batch * sizeof(double), m->stream));
+        hipLaunchKernelGGL(k_dots, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, m->stream, n, g.batch, jlo, jt,
+                           inv_s, m->curl, m->dots);
+    }
     hipLaunchKernelGGL(k_loss, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, m->stream, n, g.batch, jlo, jt, inv_s,
-                       m->curl, grad ? m->gcurl : nullptr, m->dloss, 1.0 / (double)(3 * P));
+                       m->curl, grad ? m->gcurl : nullptr, m->dloss, 1.0 / (double)(3 * P), m->lam,
+                       (const double*)m->dots);
     if (grad) {
         for (auto& L : m->L) {
             DN_CHECK(hipMemsetAsync(L.gw, 0, (size_t)L.cout * L.kpad * sizeof(float), m->stream));
@@ -670,6 +722,7 @@ int tmpg_model_create(int dim, uint64_t seed, tmpg_model** out)
         m->L.push_back(L);
     }
     rc |= dn_alloc(m, &m->dloss, 1);
+    rc |= dn_alloc(m, &m->dots, 2 * kMaxBatch);
     if (rc || prep_weights(m)) { tmpg_model_destroy(m); return TMPG_ENOMEM; }
     (void)hipStreamSynchronize(m->stream);
     *out = m;
@@ -686,6 +739,7 @@ int tmpg_model_destroy(tmpg_model* m)
     for (auto p : m->set_lo) f(p);
     for (auto p : m->set_t) f(p);
     f(m->dloss);
+    f(m->dots);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return TMPG_OK;
@@ -745,6 +799,13 @@ int tmpg_model_set_scale(tmpg_model* m, float s)
 {
     if (!m || !(s > 0.0f)) return TMPG_EINVAL;
     m->scale = s;
+    return TMPG_OK;
+}
+
+int tmpg_model_set_amplitude_weight(tmpg_model* m, float lam)
+{
+    if (!m || !(lam >= 0.0f)) return TMPG_EINVAL;
+    m->lam = lam;
     return TMPG_OK;
 }
 

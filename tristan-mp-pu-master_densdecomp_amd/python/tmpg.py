@@ -35,7 +35,7 @@ EXPORTS = [
     "tmpg_input_get_double",
     "tmpg_model_create", "tmpg_model_destroy", "tmpg_model_last_error", "tmpg_model_num_params",
     "tmpg_model_get_params", "tmpg_model_set_params", "tmpg_model_set_scale", "tmpg_model_get_scale",
-    "tmpg_model_set_base_passes", "tmpg_model_get_base_passes",
+    "tmpg_model_set_base_passes", "tmpg_model_get_base_passes", "tmpg_model_set_amplitude_weight",
     "tmpg_model_train_batch", "tmpg_model_eval", "tmpg_model_add_sample", "tmpg_model_train",
     "tmpg_denoise", "tmpg_set_denoiser",
 ]
@@ -293,6 +293,9 @@ class Model:
         s = ctypes.c_float()
         self._chk(self.lib.tmpg_model_get_scale(self.h, ctypes.byref(s)), "get_scale")
         return s.value
+
+    def set_amplitude_weight(self, lam):
+        self._chk(self.lib.tmpg_model_set_amplitude_weight(self.h, ctypes.c_float(lam)), "set_amplitude_weight")
 
     def set_base_passes(self, n):
         self._chk(self.lib.tmpg_model_set_base_passes(self.h, int(n)), "set_base_passes")

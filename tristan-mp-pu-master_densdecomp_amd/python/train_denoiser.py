@@ -226,11 +226,13 @@ def compare_runs(model, dim=2, order=2, nx=96, ny=96, nz=1, ppc_hi=32, ppc_lo=4,
     return runs
 
 
-def train(pairs, dim, iters=2000, lr=2e-3, seed=0, base_passes=0, order=2):
+def train(pairs, dim, iters=2000, lr=2e-3, seed=0, base_passes=0, order=2, amplitude_weight=0.0):
     """a model trained on (raw low-ppc, target) pairs; with base_passes > 0
-    it learns the correction of the current after that many filter passes"""
+    it learns the correction of the current after that many filter passes;
+    amplitude_weight > 0 adds the loss term on the error along the target"""
     m = tmpg.Model(dim, seed=seed)
     m.set_base_passes(base_passes)
+    m.set_amplitude_weight(amplitude_weight)
     for jl, jt in pairs:
         jin = filtered(jl, base_passes, dim, order) if base_passes > 0 else jl
         m.add_sample(jin, jt)
