@@ -71,14 +71,14 @@ def normalisation(w):
     return dict(qe=qe, qi=qi, me=me, mi=mi, qme=qe / me, qmi=qi / mi, g0=g0, ppc0=ppc0)
 
 
-def make_context(w):
+def make_context(w, sort_every=0):
     import tmpg
     nm = normalisation(w)
     ncell = w["nx"] * w["ny"] * (w["nz"] if w["dim"] == 3 else 1)
     cap = ncell * w["ppc"]
     ctx = tmpg.Context(w["dim"], w["order"], w["nx"], w["ny"], w["nz"], ntimes=w["ntimes"], c=w["c"],
                        qm=(nm["qmi"], nm["qme"]), q=(nm["qi"], nm["qe"]), mass=(nm["mi"], nm["me"]),
-                       capacity=(cap, cap))
+                       capacity=(cap, cap), sort_every=sort_every)
     for sp, ppc, gb, theta in w["load"]:
         ctx.load_uniform(sp, ppc, gb, theta, 7 + 2 * sp + (gb > 0))
     return ctx, ncell
@@ -185,6 +185,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="weibel3d_ord3_512x256x256_ppc4", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sort-every", type=int, default=0,
+                    help="steps between in-tile cell sorts of the particles (0: library default, <0: never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 times): the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters")
@@ -205,7 +207,7 @@ def main():
         sys.exit(2)
     w = WORKLOADS[a.workload]
     import tmpg
-    ctx, ncell = make_context(w)
+    ctx, ncell = make_context(w, a.sort_every)
     npart = ctx.count(0) + ctx.count(1)
     ctx.step(a.warmup)
     ctx.sync()

@@ -58,7 +58,9 @@ typedef struct tmpg_config {
     float   mass[2];      /* mi, me after normalisation (energy diagnostics) */
     int64_t capacity[2];  /* particle capacity per species (0: sized at first load) */
     int32_t tile[3];      /* cell tile for the sorted push (0: default) */
-    int32_t reserved[5];
+    int32_t sort_every;   /* tmpg_step sorts each tile's particles by cell every
+                             this many steps (0: every 4, negative: never) */
+    int32_t reserved[4];
 } tmpg_config;
 
 typedef struct tmpg_ctx tmpg_ctx;
@@ -89,6 +91,12 @@ int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n,
                        const float* u, const float* v, const float* w,
                        const float* ch, const int32_t* id);
 int tmpg_count_particles(tmpg_ctx* ctx, int species, int64_t* n);
+/* sorts each cell tile's particles by cell now (tmpg_step does this every
+ * cfg.sort_every steps).  No counterpart in the reference, whose particle
+ * order only changes at boundaries: here the order sets how many LDS bank
+ * conflicts the push's field gather meets.  TMPG_ESTATE between tmpg_move
+ * and tmpg_deposit. */
+int tmpg_sort_particles(tmpg_ctx* ctx);
 /* writes min(n_capacity, count) particles in the library's order */
 int tmpg_get_particles(tmpg_ctx* ctx, int species, int64_t n_capacity, int64_t* n,
                        float* x, float* y, float* z, float* u, float* v, float* w,

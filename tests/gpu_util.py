@@ -29,14 +29,16 @@ def norm64(p):
     return orc.normalize(np.float64, p["c"], p["c_omp"], p["gamma0"], p["ppc0"], p["me"], p["mi"])
 
 
-def make_context(tmpg, dim, order, nx, ny, nz, p, ntimes=4, compat=None, track_ids=True, capacity=(0, 0)):
+def make_context(tmpg, dim, order, nx, ny, nz, p, ntimes=4, compat=None, track_ids=True, capacity=(0, 0),
+                 sort_every=0):
     nm = norm32(p)
     kw = {}
     if compat is not None:
         kw["compat"] = compat
     return tmpg.Context(dim, order, nx, ny, nz, ntimes=ntimes, c=p["c"], corr=p["corr"],
                         qm=(float(nm["qmi"]), float(nm["qme"])), q=(float(nm["qi"]), float(nm["qe"])),
-                        mass=(float(nm["mi"]), float(nm["me"])), track_ids=track_ids, capacity=capacity, **kw)
+                        mass=(float(nm["mi"]), float(nm["me"])), track_ids=track_ids, capacity=capacity,
+                        sort_every=sort_every, **kw)
 
 
 def sorted_by_id(prt, ids):

@@ -795,6 +795,120 @@ __global__ void k_bin_scatter(PBuf in, int64_t n, PackInfo pk, Tiling tl, PBuf o
     store_part<DIM, HAS_CH, HAS_ID>(out, t * tcap + slot, pk, p);
 }
 
+// in-tile cell sort: the particles of tile t's segment of `in` are written to
+// the same segment of `out` in cell order (counting sort in LDS).  A push
+// whose consecutive lanes hold particles of the same or neighbouring cells
+// reads the staged fields with far fewer LDS bank conflicts; the push keeps
+// most of that order from step to step, the sort restores it.  Every thread
+// loads its R particles at once; each array then goes through an LDS
+// staging buffer (two, used in turn: one barrier per array), so the HBM
+// reads and writes are both coalesced.  The rank of a particle within its
+// cell is the return of its counting atomic, so particles of one cell keep
+// no particular order among themselves.
+constexpr int SORT_THREADS = 512;
+
+// R: particles per thread, for segments of up to SORT_THREADS * R particles
+template <int DIM, bool HAS_CH, bool HAS_ID, int SORT_R>
+__global__ void __launch_bounds__(SORT_THREADS) k_tile_sort(PBuf in, const int32_t* __restrict__ cnt,
+                                                           int32_t tcap, PackInfo pk, Tiling tl, PBuf out)
+{
+    extern __shared__ int sh[];
+    const int nc = tl.tx * tl.ty * tl.tz;
+    int* ccount = sh;          // [nc] particles per cell
+    int* cstart = sh + nc;     // [nc] exclusive prefix sum
+    int* stage0 = sh + 2 * nc; // [tcap] staging of one array
+    int* stage1 = stage0 + tcap;
+    const int64_t t = blockIdx.x;
+    const int ti = (int)(t % tl.ntx), tj = (int)((t / tl.ntx) % tl.nty), tk = (int)(t / ((int64_t)tl.ntx * tl.nty));
+    const int i0 = ti * tl.tx, j0 = tj * tl.ty, k0 = tk * tl.tz;
+    const int n = min(cnt[t], tcap);
+    const int64_t base = t * tcap;
+    const int tid = threadIdx.x;
+    constexpr int NA = 6 + (DIM == 3) + HAS_CH + HAS_ID;
+    int32_t v[NA][SORT_R];
+    int dst[SORT_R];
+#pragma unroll
+    for (int r = 0; r < SORT_R; ++r) {
+        const int e = tid + r * SORT_THREADS;
+        if (e < n) {
+            const int64_t s = base + e;
+            int a = 0;
+            v[a++][r] = in.cell[s];
+            v[a++][r] = __float_as_int(in.ox[s]);
+            v[a++][r] = __float_as_int(in.oy[s]);
+            if (DIM == 3) v[a++][r] = __float_as_int(in.oz[s]);
+            v[a++][r] = __float_as_int(in.u[s]);
+            v[a++][r] = __float_as_int(in.v[s]);
+            v[a++][r] = __float_as_int(in.w[s]);
+            if (HAS_CH) v[a++][r] = __float_as_int(in.ch[s]);
+            if (HAS_ID) v[a++][r] = in.id[s];
+        }
+    }
+    for (int c = tid; c < nc; c += SORT_THREADS) ccount[c] = 0;
+    __syncthreads();
+    int lc[SORT_R];
+#pragma unroll
+    for (int r = 0; r < SORT_R; ++r) {
+        lc[r] = 0;
+        dst[r] = 0;
+        if (tid + r * SORT_THREADS < n) {
+            int i, j, k;
+            unpack_cell(pk, v[0][r], i, j, k);
+            lc[r] = (i - i0) + tl.tx * ((j - j0) + tl.ty * (k - k0));
+            dst[r] = atomicAdd(ccount + lc[r], 1);
+        }
+    }
+    __syncthreads();
+    if (tid < 64) {
+        int carry = 0;
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+            const int c = c0 + tid;
+            const int x0 = c < nc ? ccount[c] : 0;
+            int x = x0;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(x, o, 64);
+                if (tid >= o) x += y;
+            }
+            if (c < nc) cstart[c] = carry + x - x0;
+            carry += __shfl(x, 63, 64);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_R; ++r)
+        if (tid + r * SORT_THREADS < n) dst[r] += cstart[lc[r]];
+    int32_t* outs[NA];
+    {
+        int a = 0;
+        outs[a++] = out.cell + base;
+        outs[a++] = reinterpret_cast<int32_t*>(out.ox + base);
+        outs[a++] = reinterpret_cast<int32_t*>(out.oy + base);
+        if (DIM == 3) outs[a++] = reinterpret_cast<int32_t*>(out.oz + base);
+        outs[a++] = reinterpret_cast<int32_t*>(out.u + base);
+        outs[a++] = reinterpret_cast<int32_t*>(out.v + base);
+        outs[a++] = reinterpret_cast<int32_t*>(out.w + base);
+        if (HAS_CH) outs[a++] = reinterpret_cast<int32_t*>(out.ch + base);
+        if (HAS_ID) outs[a++] = out.id + base;
+    }
+    // array a is scattered into stage[a % 2] and written out after the next
+    // barrier, while array a + 1 is scattered into the other buffer
+#pragma unroll
+    for (int a = 0; a <= NA; ++a) {
+        if (a < NA) {
+            int* st = (a & 1) ? stage1 : stage0;
+#pragma unroll
+            for (int r = 0; r < SORT_R; ++r)
+                if (tid + r * SORT_THREADS < n) st[dst[r]] = v[a][r];
+        }
+        if (a > 0) {
+            const int* st = ((a - 1) & 1) ? stage1 : stage0;
+            for (int e = tid; e < n; e += SORT_THREADS) outs[a - 1][e] = st[e];
+        }
+        __syncthreads();
+    }
+}
+
 // compaction: tile segments of `in` -> consecutive positions of `out`
 // starting at off[t]
 template <int DIM, bool HAS_CH, bool HAS_ID>
@@ -1060,6 +1174,40 @@ int particles_rebalance(tmpg_ctx* ctx, int s)
     rc = particles_bin(ctx, s, flat, n, sp.has_ch, sp.has_id);
     free_pbuf(flat);
     return rc;
+}
+
+int particles_sort_tiles(tmpg_ctx* ctx)
+{
+    const Grid& g = ctx->g;
+    const Tiling& tl = ctx->tl;
+    for (int s = 0; s < 2; ++s) {
+        Species& sp = ctx->sp[s];
+        if (!sp.cnt_a || sp.n == 0) continue;
+        const size_t lds = sizeof(int) * (2 * (size_t)tl.tx * tl.ty * tl.tz + 2 * (size_t)sp.tcap);
+        // the sort only speeds up the push: segments longer than one
+        // workgroup holds stay as they are
+        const int r = sp.tcap <= 4 * SORT_THREADS ? 4 : sp.tcap <= 8 * SORT_THREADS ? 8 : 16;
+        if (sp.tcap > 16 * SORT_THREADS || lds > 80 * 1024) continue;
+        dispatch(g.dim, 1, sp.has_ch, sp.has_id, [&](auto D, auto O, auto C, auto I) {
+            (void)O;
+            constexpr int DV = decltype(D)::value;
+            constexpr bool CV = decltype(C)::value, IV = decltype(I)::value;
+            const dim3 grid((unsigned)tl.ntiles), block(SORT_THREADS);
+            if (r == 4)
+                hipLaunchKernelGGL((k_tile_sort<DV, CV, IV, 4>), grid, block, lds, ctx->stream, sp.a, sp.cnt_a,
+                                   sp.tcap, ctx->pk, tl, sp.b);
+            else if (r == 8)
+                hipLaunchKernelGGL((k_tile_sort<DV, CV, IV, 8>), grid, block, lds, ctx->stream, sp.a, sp.cnt_a,
+                                   sp.tcap, ctx->pk, tl, sp.b);
+            else
+                hipLaunchKernelGGL((k_tile_sort<DV, CV, IV, 16>), grid, block, lds, ctx->stream, sp.a, sp.cnt_a,
+                                   sp.tcap, ctx->pk, tl, sp.b);
+        });
+        TMPG_HIP_CHECK(ctx, hipGetLastError());
+        // the sorted copy becomes the live buffer; counts are unchanged
+        std::swap(sp.a, sp.b);
+    }
+    return TMPG_OK;
 }
 
 int particles_check(tmpg_ctx* ctx)

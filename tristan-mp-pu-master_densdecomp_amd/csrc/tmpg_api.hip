@@ -149,6 +149,7 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
     if (!ctx) { g_create_err = "out of host memory"; return TMPG_ENOMEM; }
     ctx->cfg = *cfg;
     ctx->cfg.nz = nz;
+    ctx->sort_every = cfg->sort_every == 0 ? 4 : cfg->sort_every;
     ctx->g.dim = cfg->dim;
     ctx->g.nx = cfg->nx;
     ctx->g.ny = cfg->ny;
@@ -365,6 +366,17 @@ int tmpg_count_particles(tmpg_ctx* ctx, int species, int64_t* n)
     return TMPG_OK;
 }
 
+int tmpg_sort_particles(tmpg_ctx* ctx)
+{
+    if (!valid_ctx(ctx)) return TMPG_EINVAL;
+    if (ctx->move_pending) {
+        ctx->err = "tmpg_sort_particles: a tmpg_move awaits its tmpg_deposit";
+        return TMPG_ESTATE;
+    }
+    ctx->steps_since_sort = 0;
+    return particles_sort_tiles(ctx);
+}
+
 int tmpg_get_particles(tmpg_ctx* ctx, int species, int64_t n_capacity, int64_t* n, float* x, float* y, float* z,
                        float* u, float* v, float* w, float* ch, int32_t* id)
 {
@@ -523,6 +535,10 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
         rc = fields_bhalf(ctx);
         if (!rc) rc = fields_zero_j(ctx);
         (void)hipEventRecord(ev[1], ctx->stream);
+        if (!rc && ctx->sort_every > 0 && ++ctx->steps_since_sort >= ctx->sort_every) {
+            ctx->steps_since_sort = 0;
+            rc = particles_sort_tiles(ctx);
+        }
         if (!rc) rc = pre_push(ctx);
         ctx->push_events = ev + 6;   // around the tiled push kernel itself
         if (!rc) rc = particles_push(ctx, 0);

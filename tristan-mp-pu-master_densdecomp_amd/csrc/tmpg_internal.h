@@ -114,6 +114,8 @@ struct tmpg_ctx {
     std::vector<hipEvent_t> prof;  // 6 events per tmpg_step iteration since the last read
     int prof_steps = 0;
     hipEvent_t* push_events = nullptr;  // set during tmpg_step: recorded around the tiled push kernel
+    int sort_every = 4;                 // tmpg_step sorts each tile's particles by cell every this many steps
+    int steps_since_sort = 0;
     std::string err;
 };
 
@@ -148,6 +150,7 @@ void free_species(Species& s);
 int particles_load_uniform(tmpg_ctx* ctx, int s, int ppc, float drift, float theta, uint64_t seed);
 // apply fn to every live particle slot: kernels over the tiled layout use this
 int particles_check(tmpg_ctx* ctx);
+int particles_sort_tiles(tmpg_ctx* ctx);
 int diag_energies(tmpg_ctx* ctx, double out[4]);
 int diag_rho(tmpg_ctx* ctx, double* rho_dev);
 int diag_continuity(tmpg_ctx* ctx, double* maxres, double* rhonorm);

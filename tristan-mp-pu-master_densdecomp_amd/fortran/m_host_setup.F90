@@ -120,6 +120,7 @@ contains
 		cfg%mass = [real(hp%mass_i, c_float), real(hp%mass_e, c_float)]
 		cfg%capacity = 0
 		cfg%tile = 0
+		cfg%sort_every = 0
 		cfg%reserved = 0
 	end subroutine make_config
 

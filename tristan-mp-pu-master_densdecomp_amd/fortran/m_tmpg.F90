@@ -19,7 +19,8 @@ module m_tmpg
 		real(c_float) :: qm(2), q(2), mass(2)
 		integer(c_int64_t) :: capacity(2)
 		integer(c_int32_t) :: tile(3)
-		integer(c_int32_t) :: reserved(5)
+		integer(c_int32_t) :: sort_every
+		integer(c_int32_t) :: reserved(4)
 	end type tmpg_config
 
 	interface
@@ -61,6 +62,10 @@ module m_tmpg
 			type(c_ptr), value :: ctx
 			integer(c_int), value :: species
 			integer(c_int64_t), intent(out) :: n
+		end function
+		integer(c_int) function tmpg_sort_particles(ctx) bind(C, name="tmpg_sort_particles")
+			import :: c_int, c_ptr
+			type(c_ptr), value :: ctx
 		end function
 		! per-step kernels
 		integer(c_int) function tmpg_advance_b_half(ctx) bind(C, name="tmpg_advance_b_half")  ! advance_Bhalfstep

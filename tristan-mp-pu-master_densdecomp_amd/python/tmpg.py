@@ -27,7 +27,7 @@ COMPAT_DEFAULT = 3
 EXPORTS = [
     "tmpg_create", "tmpg_destroy", "tmpg_last_error", "tmpg_sync",
     "tmpg_set_fields", "tmpg_get_fields", "tmpg_set_currents", "tmpg_get_currents",
-    "tmpg_set_particles", "tmpg_count_particles", "tmpg_get_particles",
+    "tmpg_set_particles", "tmpg_count_particles", "tmpg_get_particles", "tmpg_sort_particles",
     "tmpg_advance_b_half", "tmpg_advance_e", "tmpg_reset_currents", "tmpg_move", "tmpg_deposit",
     "tmpg_push_deposit", "tmpg_fold_currents", "tmpg_filter", "tmpg_add_current", "tmpg_step",
     "tmpg_diag", "tmpg_charge_density", "tmpg_track_continuity", "tmpg_load_uniform",
@@ -58,7 +58,8 @@ class Config(ctypes.Structure):
         ("mass", ctypes.c_float * 2),
         ("capacity", ctypes.c_int64 * 2),
         ("tile", ctypes.c_int32 * 3),
-        ("reserved", ctypes.c_int32 * 5),
+        ("sort_every", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -99,7 +100,7 @@ class Context:
 
     def __init__(self, dim, order, nx, ny, nz=1, ntimes=4, compat=COMPAT_DEFAULT, c=0.45, corr=1.025,
                  qm=(1.0, -1.0), q=(1.0, -1.0), mass=(1.0, 1.0), capacity=(0, 0), track_ids=False,
-                 tile=(0, 0, 0)):
+                 tile=(0, 0, 0), sort_every=0):
         self.lib = load_library()
         cfg = Config()
         cfg.dim, cfg.order, cfg.nx, cfg.ny, cfg.nz = dim, order, nx, ny, (nz if dim == 3 else 1)
@@ -109,6 +110,7 @@ class Context:
             cfg.qm[s], cfg.q[s], cfg.mass[s], cfg.capacity[s] = qm[s], q[s], mass[s], capacity[s]
         for a in range(3):
             cfg.tile[a] = tile[a]
+        cfg.sort_every = sort_every
         self.cfg = cfg
         self.dim, self.order = dim, order
         self.nx, self.ny, self.nz = nx, ny, (nz if dim == 3 else 1)
@@ -183,6 +185,9 @@ class Context:
         self._chk(self.lib.tmpg_get_particles(self.h, species, ctypes.c_int64(n), ctypes.byref(got),
                                               *(_p(out[r]) for r in range(7)), _p(ids)), "get_particles")
         return (out, ids) if with_ids else out
+
+    def sort_particles(self):
+        self._chk(self.lib.tmpg_sort_particles(self.h), "sort_particles")
 
     def load_uniform(self, species, ppc, drift_gb, theta, seed):
         self._chk(self.lib.tmpg_load_uniform(self.h, species, ppc, ctypes.c_float(drift_gb),
