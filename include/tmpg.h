@@ -91,11 +91,12 @@ int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n,
                        const float* u, const float* v, const float* w,
                        const float* ch, const int32_t* id);
 int tmpg_count_particles(tmpg_ctx* ctx, int species, int64_t* n);
-/* sorts each cell tile's particles by cell now (tmpg_step does this every
- * cfg.sort_every steps).  No counterpart in the reference, whose particle
- * order only changes at boundaries: here the order sets how many LDS bank
- * conflicts the push's field gather meets.  TMPG_ESTATE between tmpg_move
- * and tmpg_deposit. */
+/* [reorder_particles] code/particles.F90:394-497 (a counting sort of each
+ * species by cell every 10 laps, called from code/tristanmainloop.F90:303):
+ * sorts each cell tile's particles by cell now; tmpg_step does this every
+ * cfg.sort_every steps.  The order sets how many LDS bank conflicts the
+ * push's field gather meets.  TMPG_ESTATE between tmpg_move and
+ * tmpg_deposit. */
 int tmpg_sort_particles(tmpg_ctx* ctx);
 /* writes min(n_capacity, count) particles in the library's order */
 int tmpg_get_particles(tmpg_ctx* ctx, int species, int64_t n_capacity, int64_t* n,
