@@ -86,7 +86,11 @@ __device__ __forceinline__ float gelu_grad(float z)
     return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * k0 * (1.0f + 3.0f * k1 * z * z);
 }
 
-template <int NT>   // NT = cout_pad / 16
+// One wave computes MT tiles of 16 pixels x cout_pad channels; 4 waves per
+// block.  A lane's pixel coordinates are found once; each K step then only
+// shifts them by the tap's offset (periodic wrap by compare), and the B
+// operand (weights) of a K step serves all MT pixel tiles.
+template <int NT, int MT>   // NT = cout_pad / 16
 __global__ void __launch_bounds__(256) k_conv(Geo g, const uint16_t* __restrict__ in, int cin, int taps,
                                               const uint16_t* __restrict__ W, int kpad,
                                               const float* __restrict__ bias, int cout,
@@ -95,50 +99,81 @@ __global__ void __launch_bounds__(256) k_conv(Geo g, const uint16_t* __restrict_
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t P = g.npix * g.batch;
-    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * MT);
     if (p0 >= P) return;
     const int i = lane & 15, kq = lane >> 4;
-    const int64_t p = p0 + i;
-    const bool pv = p < P;
-    f32x4 acc[NT];
+    int px[MT], py[MT], pz[MT];
+    int64_t base[MT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n) acc[n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int mt = 0; mt < MT; ++mt) {
+        const int64_t p = p0 + mt * 16 + i;
+        px[mt] = -1;   // marks a pixel beyond P
+        py[mt] = pz[mt] = 0;
+        base[mt] = 0;
+        if (p < P) {
+            const int64_t img = p / g.npix;
+            int r = (int)(p - img * g.npix);
+            px[mt] = r % g.nx;
+            r /= g.nx;
+            py[mt] = r % g.ny;
+            pz[mt] = r / g.ny;
+            base[mt] = img * g.npix;
+        }
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[mt][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const int K = taps * cin;
     for (int k0 = 0; k0 < kpad; k0 += 16) {
         const int k = k0 + 4 * kq;
-        bf16x4 a = bf16x4{0, 0, 0, 0};
-        if (pv && k < K) {
-            const int tap = k / cin, ci = k - tap * cin;
-            const int64_t q = neighbour(g, p, tap);
-            const uint2 v = *reinterpret_cast<const uint2*>(in + q * cin + ci);
-            a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16);
-            a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
-        }
+        const bool kv = k < K;
+        const int tap = kv ? k / cin : 0, ci = k - tap * cin;
+        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = (g.dim == 3) ? tap / 9 - 1 : 0;
+        bf16x4 b[NT];
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
             const int co = n * 16 + i;
             const uint2 w = *reinterpret_cast<const uint2*>(W + (int64_t)co * kpad + k);
-            bf16x4 b;
-            b[0] = (short)(w.x & 0xffff); b[1] = (short)(w.x >> 16);
-            b[2] = (short)(w.y & 0xffff); b[3] = (short)(w.y >> 16);
-            acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+            b[n][0] = (short)(w.x & 0xffff); b[n][1] = (short)(w.x >> 16);
+            b[n][2] = (short)(w.y & 0xffff); b[n][3] = (short)(w.y >> 16);
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            bf16x4 a = bf16x4{0, 0, 0, 0};
+            if (kv && px[mt] >= 0) {
+                int x = px[mt] + dx, y = py[mt] + dy, z = pz[mt] + dz;
+                x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+                y = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
+                z = z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z);
+                const int64_t q = base[mt] + (x + g.nx * (y + g.ny * z));
+                const uint2 v = *reinterpret_cast<const uint2*>(in + q * cin + ci);
+                a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16);
+                a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
+            }
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+                acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[n], acc[mt][n], 0, 0, 0);
         }
     }
     // D[i][j]: j = lane % 16 (channel), i = 4*(lane/16) + r (pixel)
 #pragma unroll
-    for (int n = 0; n < NT; ++n) {
-        const int co = n * 16 + (lane & 15);
-        if (co >= cout) continue;
-        const float bv = bias ? bias[co] : 0.0f;
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t pp = p0 + 4 * kq + r;
-            if (pp >= P) continue;
-            const float z = acc[n][r] + bv;
-            if (raw_out) raw_out[pp * cout + co] = z;
-            if (gelu_out) gelu_out[pp * gelu_stride + co] = f2bf(gelu(z));
+        for (int n = 0; n < NT; ++n) {
+            const int co = n * 16 + (lane & 15);
+            if (co >= cout) continue;
+            const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t pp = p0 + mt * 16 + 4 * kq + r;
+                if (pp >= P) continue;
+                const float zv = acc[mt][n][r] + bv;
+                if (raw_out) raw_out[pp * cout + co] = zv;
+                if (gelu_out) gelu_out[pp * gelu_stride + co] = f2bf(gelu(zv));
+            }
         }
-    }
 }
 
 // dW[co][k] += sum_p dz[p][co] * patch(in)[p][k], db[co] += sum_p dz[p][co].
@@ -541,25 +576,27 @@ void launch_conv(tmpg_model* m, const Geo& g, const uint16_t* in, int cin, int t
                  int cout_pad, const float* bias, int cout, float* raw, uint16_t* gout, int gstride)
 {
     const int64_t P = g.npix * g.batch;
-    dim3 grid((unsigned)cdiv(P, 64));
+    constexpr int MT = 4;
+    dim3 grid((unsigned)cdiv(P, 64 * MT));
     if (cout_pad == 16)
-        hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), 0, m->stream, g, in, cin, taps, W, kpad, bias, cout, raw,
-                           gout, gstride);
+        hipLaunchKernelGGL((k_conv<1, MT>), grid, dim3(256), 0, m->stream, g, in, cin, taps, W, kpad, bias, cout,
+                           raw, gout, gstride);
     else
-        hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, m->stream, g, in, cin, taps, W, kpad, bias, cout, raw,
-                           gout, gstride);
+        hipLaunchKernelGGL((k_conv<2, MT>), grid, dim3(256), 0, m->stream, g, in, cin, taps, W, kpad, bias, cout,
+                           raw, gout, gstride);
 }
 
-// forward from m->x0: fills z, h, A and curl
-void forward(tmpg_model* m, const Geo& g)
+// forward from m->x0: fills h, A and curl, and the pre-activations z that
+// the backward pass needs when keep_z
+void forward(tmpg_model* m, const Geo& g, bool keep_z = true)
 {
     const uint16_t* in = m->x0;
     int cin = 4;
     for (int l = 0; l < m->nlayers; ++l) {
         DLayer& L = m->L[l];
         const bool last = l == m->nlayers - 1;
-        launch_conv(m, g, in, cin, L.taps, L.wf, L.kpad, L.cout_pad, L.b, L.cout, last ? m->A : m->z[l],
-                    last ? nullptr : m->h[l], L.cout);
+        launch_conv(m, g, in, cin, L.taps, L.wf, L.kpad, L.cout_pad, L.b, L.cout,
+                    last ? m->A : (keep_z ? m->z[l] : nullptr), last ? nullptr : m->h[l], L.cout);
         if (!last) { in = m->h[l]; cin = L.cout; }
     }
     const int64_t P = g.npix * g.batch;
@@ -955,7 +992,7 @@ int denoise_apply(tmpg_ctx* ctx, tmpg_model* m)
     const int64_t n = g.npix;
     hipLaunchKernelGGL(k_pack_input3, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, ctx->f[JX],
                        ctx->f[JY], ctx->f[JZ], n, 1.0f / m->scale, m->x0);
-    forward(m, g);
+    forward(m, g, false);
     hipLaunchKernelGGL(k_apply, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, n, m->curl, m->scale,
                        ctx->f[JX], ctx->f[JY], ctx->f[JZ]);
     TMPG_HIP_CHECK(ctx, hipGetLastError());

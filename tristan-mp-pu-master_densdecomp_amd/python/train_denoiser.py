@@ -111,6 +111,48 @@ def make_pairs(dim=2, order=2, nx=128, ny=128, nz=1, ppc_hi=64, ppc_lo=4, gb=0.5
     return pairs
 
 
+def make_pairs_onpolicy(model, dim=2, order=2, nx=96, ny=96, nz=1, ppc_hi=32, ppc_lo=4, gb=0.5, theta=2.5e-3,
+                        steps=600, every=30, first=10, seed=0, c_omp=5.0, ntimes=4):
+    """pairs from the states a denoised low-ppc run visits.  The low-ppc
+    particles (every k-th particle of a high-ppc load, charge and mass x k)
+    drive the fields with their denoised current (the filtered one when
+    model is None); the high-ppc set moves in the same fields and its
+    filtered current is the target.  Both contexts hold identical fields at
+    every step, so each low-ppc particle follows its high-ppc copy."""
+    k = ppc_hi // ppc_lo
+    hi, _ = make_weibel(dim, order, nx, ny, nz, ppc_hi, gb, theta, seed, c_omp=c_omp, ntimes=ntimes)
+    lo, _ = make_weibel(dim, order, nx, ny, nz, ppc_hi, gb, theta, seed, c_omp=c_omp, ntimes=ntimes,
+                        q_scale=float(k), load=False)
+    lo.set_fields(hi.get_fields())
+    for s in range(2):
+        lo.set_particles(s, hi.get_particles(s)[:, ::k])
+    pairs = []
+    for step in range(steps):
+        cap = step >= first and (step - first) % every == 0
+        for c in (hi, lo):
+            c.advance_b_half()
+            c.reset_currents()
+            c.push_deposit()
+        if cap:
+            j_lo = interior(lo, lo.get_currents())
+        if model is not None:
+            lo.denoise(model)
+        else:
+            lo.filter(ntimes)
+        j_used = lo.get_currents()
+        if cap:
+            hi.filter(ntimes)
+            pairs.append((j_lo, interior(hi, hi.get_currents())))
+        hi.set_currents(j_used)
+        for c in (hi, lo):
+            c.advance_b_half()
+            c.advance_e()
+            c.add_current()
+    hi.close()
+    lo.close()
+    return pairs
+
+
 def helmholtz(j, dim):
     """split a periodic Yee-grid current (3, nz, ny, nx) into its
     longitudinal part grad_f(phi), with div_b grad_f phi = div_b J, and its
