@@ -14,8 +14,11 @@
 // the weight gradient is its own MFMA kernel reducing over pixels.
 // Training (Adam) runs on the device; master weights are FP32.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "tmpg_internal.h"
@@ -70,8 +73,7 @@ __device__ __forceinline__ int64_t neighbour(const Geo& g, int64_t p, int tap)
 // out[p][co] = bias[co] + sum_{tap,ci} W[co][tap*cin + ci] * in[nb(p,tap)][ci]
 // in: BF16 [P][cin] (cin multiple of 4); W: BF16 [cout_pad][kpad] with
 // kpad = taps*cin rounded up to 16 (zeros beyond); out: FP32 [P][cout]
-// (cout <= cout_pad, multiple of 16 for cout_pad).  One wave computes 16
-// pixels x cout_pad channels; 4 waves per block.
+// (cout <= cout_pad, multiple of 16 for cout_pad).
 // gelu_out (optional): BF16 gelu(out) [P][cout]; raw_out (optional): FP32.
 __device__ __forceinline__ float gelu(float z)
 {
@@ -86,80 +88,113 @@ __device__ __forceinline__ float gelu_grad(float z)
     return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * k0 * (1.0f + 3.0f * k1 * z * z);
 }
 
-// One wave computes MT tiles of 16 pixels x cout_pad channels; 4 waves per
-// block.  A lane's pixel coordinates are found once; each K step then only
-// shifts them by the tap's offset (periodic wrap by compare), and the B
-// operand (weights) of a K step serves all MT pixel tiles.
-template <int NT, int MT>   // NT = cout_pad / 16
-__global__ void __launch_bounds__(256) k_conv(Geo g, const uint16_t* __restrict__ in, int cin, int taps,
-                                              const uint16_t* __restrict__ W, int kpad,
+// A workgroup computes one tile of TX x TY x TZ pixels (2D 64x4, 3D 16x4x4)
+// for all cout_pad channels.  Its weights (KS K steps of 16) are loaded into
+// registers first, so their latency hides behind the staging of the tile's
+// input with a one-pixel periodic halo in LDS.  The LDS pixel stride is
+// CIN + 4 halfwords for CIN >= 16, so the 16 pixels of an MFMA row group
+// fall in distinct banks.  Each of the 4 waves owns 4 row segments of 16
+// pixels (the MFMA M dimension); a K step of 16 (4 channels x 4 lane
+// groups) reads its A operands from LDS, and its weights serve all 4
+// segments.
+template <int DIM>
+struct ConvTile {
+    static constexpr int TX = DIM == 3 ? 16 : 64, TY = 4, TZ = DIM == 3 ? 4 : 1;
+    static constexpr int HX = TX + 2, HY = TY + 2, HZ = DIM == 3 ? TZ + 2 : 1;
+    static constexpr int MPR = TX / 16;   // 16-pixel segments per row
+};
+
+__host__ __device__ constexpr int conv_pixel_stride(int cin) { return cin + (cin >= 16 ? 4 : 0); }
+
+__device__ __forceinline__ int wrap_once(int x, int n)
+{
+    x += x < 0 ? n : 0;
+    if (x >= n) x -= n;
+    if (x >= n) x %= n;   // a tile wider than the grid
+    return x;
+}
+
+template <int NT, int DIM, int CIN>   // NT = cout_pad / 16
+__global__ void __launch_bounds__(256) k_conv(Geo g, const uint16_t* __restrict__ in,
+                                              const uint16_t* __restrict__ W,
                                               const float* __restrict__ bias, int cout,
                                               float* __restrict__ raw_out, uint16_t* __restrict__ gelu_out,
                                               int gelu_stride)
 {
+    using T = ConvTile<DIM>;
+    constexpr int TAPS = DIM == 3 ? 27 : 9, K = TAPS * CIN, KS = (K + 15) / 16, KPAD = KS * 16;
+    constexpr int PS = conv_pixel_stride(CIN), CU = CIN / 4;
+    extern __shared__ uint16_t sm[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t P = g.npix * g.batch;
-    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * MT);
-    if (p0 >= P) return;
     const int i = lane & 15, kq = lane >> 4;
-    int px[MT], py[MT], pz[MT];
-    int64_t base[MT];
+    bf16x4 bw[KS][NT];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const int64_t p = p0 + mt * 16 + i;
-        px[mt] = -1;   // marks a pixel beyond P
-        py[mt] = pz[mt] = 0;
-        base[mt] = 0;
-        if (p < P) {
-            const int64_t img = p / g.npix;
-            int r = (int)(p - img * g.npix);
-            px[mt] = r % g.nx;
-            r /= g.nx;
-            py[mt] = r % g.ny;
-            pz[mt] = r / g.ny;
-            base[mt] = img * g.npix;
-        }
-    }
-    f32x4 acc[MT][NT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int n = 0; n < NT; ++n) acc[mt][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int K = taps * cin;
-    for (int k0 = 0; k0 < kpad; k0 += 16) {
-        const int k = k0 + 4 * kq;
-        const bool kv = k < K;
-        const int tap = kv ? k / cin : 0, ci = k - tap * cin;
-        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = (g.dim == 3) ? tap / 9 - 1 : 0;
-        bf16x4 b[NT];
+    for (int s = 0; s < KS; ++s)
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
-            const int co = n * 16 + i;
-            const uint2 w = *reinterpret_cast<const uint2*>(W + (int64_t)co * kpad + k);
-            b[n][0] = (short)(w.x & 0xffff); b[n][1] = (short)(w.x >> 16);
-            b[n][2] = (short)(w.y & 0xffff); b[n][3] = (short)(w.y >> 16);
+            const uint2 w = *reinterpret_cast<const uint2*>(W + (n * 16 + i) * KPAD + s * 16 + 4 * kq);
+            bw[s][n][0] = (short)(w.x & 0xffff); bw[s][n][1] = (short)(w.x >> 16);
+            bw[s][n][2] = (short)(w.y & 0xffff); bw[s][n][3] = (short)(w.y >> 16);
         }
+    const int ntx = (g.nx + T::TX - 1) / T::TX, nty = (g.ny + T::TY - 1) / T::TY;
+    const int ntz = (g.nz + T::TZ - 1) / T::TZ;
+    int64_t bid = blockIdx.x;
+    const int tx = (int)(bid % ntx);
+    bid /= ntx;
+    const int ty = (int)(bid % nty);
+    bid /= nty;
+    const int tz = (int)(bid % ntz);
+    const int64_t ibase = (bid / ntz) * g.npix;
+    const int x0 = tx * T::TX, y0 = ty * T::TY, z0 = tz * T::TZ;
+    // stage the haloed tile: units of 4 channels (8 bytes)
+    constexpr int NUNITS = T::HX * T::HY * T::HZ * CU;
+    for (int u = threadIdx.x; u < NUNITS; u += 256) {
+        const int pix = u / CU, c4 = u - pix * CU;
+        const int hx = pix % T::HX, hy = (pix / T::HX) % T::HY, hz = pix / (T::HX * T::HY);
+        const int x = wrap_once(x0 + hx - 1, g.nx), y = wrap_once(y0 + hy - 1, g.ny);
+        const int z = DIM == 3 ? wrap_once(z0 + hz - 1, g.nz) : 0;
+        const int64_t q = ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+        *reinterpret_cast<uint2*>(sm + pix * PS + 4 * c4) = *reinterpret_cast<const uint2*>(in + q * CIN + 4 * c4);
+    }
+    __syncthreads();
+    int cen[4];   // LDS pixel of this lane's A row in each segment
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
+    for (int mt = 0; mt < 4; ++mt) {
+        const int m = wave * 4 + mt, row = m / T::MPR, lx = (m % T::MPR) * 16 + i;
+        const int ly = row % T::TY, lz = row / T::TY;
+        cen[mt] = (((DIM == 3 ? lz + 1 : 0) * T::HY + ly + 1) * T::HX + lx + 1) * PS;
+    }
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[mt][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int k = s * 16 + 4 * kq;
+        const bool kv = k < K;
+        const int tap = kv ? k / CIN : 0, ci = k - tap * CIN;
+        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = (DIM == 3) ? tap / 9 - 1 : 0;
+        const int doff = ((dz * T::HY + dy) * T::HX + dx) * PS + ci;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
             bf16x4 a = bf16x4{0, 0, 0, 0};
-            if (kv && px[mt] >= 0) {
-                int x = px[mt] + dx, y = py[mt] + dy, z = pz[mt] + dz;
-                x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
-                y = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
-                z = z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z);
-                const int64_t q = base[mt] + (x + g.nx * (y + g.ny * z));
-                const uint2 v = *reinterpret_cast<const uint2*>(in + q * cin + ci);
+            if (kv) {
+                const uint2 v = *reinterpret_cast<const uint2*>(sm + cen[mt] + doff);
                 a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16);
                 a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
             }
 #pragma unroll
             for (int n = 0; n < NT; ++n)
-                acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[n], acc[mt][n], 0, 0, 0);
+                acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[s][n], acc[mt][n], 0, 0, 0);
         }
     }
     // D[i][j]: j = lane % 16 (channel), i = 4*(lane/16) + r (pixel)
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < 4; ++mt) {
+        const int m = wave * 4 + mt, row = m / T::MPR;
+        const int y = y0 + row % T::TY, z = z0 + row / T::TY;
+        if (y >= g.ny || z >= g.nz) continue;
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
             const int co = n * 16 + (lane & 15);
@@ -167,13 +202,15 @@ __global__ void __launch_bounds__(256) k_conv(Geo g, const uint16_t* __restrict_
             const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t pp = p0 + mt * 16 + 4 * kq + r;
-                if (pp >= P) continue;
+                const int x = x0 + (m % T::MPR) * 16 + 4 * kq + r;
+                if (x >= g.nx) continue;
+                const int64_t pp = ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
                 const float zv = acc[mt][n][r] + bv;
                 if (raw_out) raw_out[pp * cout + co] = zv;
                 if (gelu_out) gelu_out[pp * gelu_stride + co] = f2bf(gelu(zv));
             }
         }
+    }
 }
 
 // dW[co][k] += sum_p dz[p][co] * patch(in)[p][k], db[co] += sum_p dz[p][co].
@@ -572,39 +609,66 @@ int prep_weights(tmpg_model* m)
     return TMPG_OK;
 }
 
-void launch_conv(tmpg_model* m, const Geo& g, const uint16_t* in, int cin, int taps, const uint16_t* W, int kpad,
-                 int cout_pad, const float* bias, int cout, float* raw, uint16_t* gout, int gstride)
+int launch_conv(tmpg_model* m, const Geo& g, const uint16_t* in, int cin, int taps, const uint16_t* W, int kpad,
+                int cout_pad, const float* bias, int cout, float* raw, uint16_t* gout, int gstride)
 {
-    const int64_t P = g.npix * g.batch;
-    constexpr int MT = 4;
-    dim3 grid((unsigned)cdiv(P, 64 * MT));
-    if (cout_pad == 16)
-        hipLaunchKernelGGL((k_conv<1, MT>), grid, dim3(256), 0, m->stream, g, in, cin, taps, W, kpad, bias, cout,
-                           raw, gout, gstride);
-    else
-        hipLaunchKernelGGL((k_conv<2, MT>), grid, dim3(256), 0, m->stream, g, in, cin, taps, W, kpad, bias, cout,
-                           raw, gout, gstride);
+    auto go = [&](auto dim, auto nt, auto ci) {
+        constexpr int D = decltype(dim)::value, NT = decltype(nt)::value, CIN = decltype(ci)::value;
+        using T = ConvTile<D>;
+        const int64_t tiles = (int64_t)cdiv(g.nx, T::TX) * cdiv(g.ny, T::TY) * cdiv(g.nz, T::TZ) * g.batch;
+        const size_t lds = sizeof(uint16_t) * (size_t)T::HX * T::HY * T::HZ * conv_pixel_stride(CIN);
+        hipLaunchKernelGGL((k_conv<NT, D, CIN>), dim3((unsigned)tiles), dim3(256), lds, m->stream, g, in, W, bias,
+                           cout, raw, gout, gstride);
+    };
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using C4 = std::integral_constant<int, 4>;
+    using C16 = std::integral_constant<int, 16>;
+    using C32 = std::integral_constant<int, 32>;
+    // the layer shapes of the two networks: inputs of 4 channels (J) or the
+    // width (2D 32, 3D 16), and 16 or 32 output channels after padding
+    auto by_cin = [&](auto dim, auto nt) {
+        if (cin == 4) go(dim, nt, C4{});
+        else if constexpr (decltype(dim)::value == 3) go(dim, nt, C16{});
+        else go(dim, nt, C32{});
+    };
+    auto by_nt = [&](auto dim) {
+        if (cout_pad == 16) by_cin(dim, I1{}); else by_cin(dim, I2{});
+    };
+    const int width = g.dim == 3 ? 16 : 32;
+    const int taps_dim = g.dim == 3 ? 27 : 9;
+    if ((cin != 4 && cin != width) || taps != taps_dim || kpad != (taps * cin + 15) / 16 * 16 ||
+        (cout_pad != 16 && cout_pad != 32)) {
+        // unreachable for the two networks built here
+        m->err = "no conv kernel for cin " + std::to_string(cin) + ", cout_pad " + std::to_string(cout_pad);
+        return TMPG_EINVAL;
+    }
+    if (g.dim == 3) by_nt(I3{}); else by_nt(I2{});
+    return TMPG_OK;
 }
 
 // forward from m->x0: fills h, A and curl, and the pre-activations z that
 // the backward pass needs when keep_z
-void forward(tmpg_model* m, const Geo& g, bool keep_z = true)
+int forward(tmpg_model* m, const Geo& g, bool keep_z = true)
 {
     const uint16_t* in = m->x0;
     int cin = 4;
     for (int l = 0; l < m->nlayers; ++l) {
         DLayer& L = m->L[l];
         const bool last = l == m->nlayers - 1;
-        launch_conv(m, g, in, cin, L.taps, L.wf, L.kpad, L.cout_pad, L.b, L.cout,
-                    last ? m->A : (keep_z ? m->z[l] : nullptr), last ? nullptr : m->h[l], L.cout);
+        const int rc = launch_conv(m, g, in, cin, L.taps, L.wf, L.kpad, L.cout_pad, L.b, L.cout,
+                                   last ? m->A : (keep_z ? m->z[l] : nullptr), last ? nullptr : m->h[l], L.cout);
+        if (rc) return rc;
         if (!last) { in = m->h[l]; cin = L.cout; }
     }
     const int64_t P = g.npix * g.batch;
     hipLaunchKernelGGL(k_curl, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, m->stream, g, m->A, m->curl);
+    return TMPG_OK;
 }
 
 // backward from m->gcurl (dL/dcurl): accumulates gw, gb
-void backward(tmpg_model* m, const Geo& g)
+int backward(tmpg_model* m, const Geo& g)
 {
     const int64_t P = g.npix * g.batch;
     const int64_t chunk = 1024;
@@ -621,12 +685,15 @@ void backward(tmpg_model* m, const Geo& g)
                            L.cout, L.cout_pad, in, cin, L.taps, L.kpad, chunk, L.gw, L.gb);
         if (l == 0) break;
         // input gradient: conv of dz with the transposed, flipped weights
-        launch_conv(m, g, m->dz, dz_c, L.taps, L.wt, L.kpadT, L.cinT_pad, nullptr, L.cin, m->dh, nullptr, 0);
+        const int rc = launch_conv(m, g, m->dz, dz_c, L.taps, L.wt, L.kpadT, L.cinT_pad, nullptr, L.cin, m->dh,
+                                   nullptr, 0);
+        if (rc) return rc;
         const int c = m->L[l - 1].cout;
         hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)cdiv(P * c, 256)), dim3(256), 0, m->stream, P * c, m->dh,
                            m->z[l - 1], m->dz);
         dz_c = c;
     }
+    return TMPG_OK;
 }
 
 int set_geometry(tmpg_model* m, Geo& g, int nx, int ny, int nz, int batch)
@@ -647,7 +714,8 @@ int run_batch(tmpg_model* m, const Geo& g, const float* jlo, const float* jt, bo
     const float inv_s = 1.0f / m->scale;
     hipLaunchKernelGGL(k_pack_input, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, m->stream, jlo, n, g.batch, inv_s,
                        m->x0);
-    forward(m, g);
+    int rc = forward(m, g);
+    if (rc) return rc;
     DN_CHECK(hipMemsetAsync(m->dloss, 0, sizeof(double), m->stream));
     if (m->lam > 0.0f) {
         if (g.batch > kMaxBatch) { m->err = "batch too large for the amplitude term"; return TMPG_EINVAL; }
@@ -663,7 +731,8 @@ int run_batch(tmpg_model* m, const Geo& g, const float* jlo, const float* jt, bo
             DN_CHECK(hipMemsetAsync(L.gw, 0, (size_t)L.cout * L.kpad * sizeof(float), m->stream));
             DN_CHECK(hipMemsetAsync(L.gb, 0, (size_t)L.cout * sizeof(float), m->stream));
         }
-        backward(m, g);
+        rc = backward(m, g);
+        if (rc) return rc;
     }
     DN_CHECK(hipGetLastError());
     if (loss) {
@@ -992,7 +1061,8 @@ int denoise_apply(tmpg_ctx* ctx, tmpg_model* m)
     const int64_t n = g.npix;
     hipLaunchKernelGGL(k_pack_input3, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, ctx->f[JX],
                        ctx->f[JY], ctx->f[JZ], n, 1.0f / m->scale, m->x0);
-    forward(m, g, false);
+    rc = forward(m, g, false);
+    if (rc) { ctx->err = m->err; return rc; }
     hipLaunchKernelGGL(k_apply, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, n, m->curl, m->scale,
                        ctx->f[JX], ctx->f[JY], ctx->f[JZ]);
     TMPG_HIP_CHECK(ctx, hipGetLastError());
