@@ -17,6 +17,13 @@ method's and the network improves the transverse part.
 
     python train_denoiser.py --dim 2 --order 2 --nx 256 --ny 256 --ppc-hi 64 --ppc-lo 4 \
         --steps 400 --every 20 --seeds 0 1 2 3 --iters 3000 --out model.npz
+
+The recipe that reproduces a 32-ppc pair Weibel run from 2 ppc per species
+(8 filter passes under the correction, amplitude weight 8, pairs from three
+96x96 runs), judged on 1024x1024 cells:
+
+    python train_denoiser.py --nx 96 --ny 96 --ppc-hi 32 --ppc-lo 2 --steps 600 --every 30 \
+        --seeds 0 1 2 --iters 3000 --study 1024
 """
 import argparse
 import os
@@ -268,6 +275,43 @@ def compare_runs(model, dim=2, order=2, nx=96, ny=96, nz=1, ppc_hi=32, ppc_lo=4,
     return runs
 
 
+def weibel_run(dim, order, n, ppc, seed, model=None, laps=600, every=10, ppc_norm=32, gb=0.5, theta=2.5e-3,
+               c_omp=5.0, ntimes=4):
+    """one 2D pair Weibel run on n x n cells at `ppc` particles per cell per
+    species (charge and mass scaled to the density of `ppc_norm`), with the
+    denoiser in place of the filter when `model` is given.  Returns the
+    B-energy growth rate, the peak B energy and the GPU time per lap."""
+    ctx, _ = make_weibel(dim, order, n, n, 1, ppc_norm, gb, theta, seed, c_omp=c_omp, ntimes=ntimes,
+                         q_scale=ppc_norm / ppc, load=False)
+    for k, (s, d) in enumerate(((0, -gb), (1, -gb), (0, gb), (1, gb))):
+        ctx.load_uniform(s, ppc // 2, d, theta, 4 * seed + k + 1)
+    if model is not None:
+        ctx.set_denoiser(model)
+    ctx.step(1)
+    ctx.sync()
+    ctx.last_timings()
+    laps_, hist = energy_history(ctx, laps, every)
+    ms = ctx.last_timings()
+    ctx.close()
+    eb = hist[:, 1]
+    return growth_rate(laps_, eb, lo_frac=1e-2, hi_frac=0.3), float(eb.max()), float(ms[4] / max(ms[3], 1))
+
+
+def parity_study(model, n=192, seeds=range(11, 19), ppc_hi=32, ppc_lo=2, laps=600, order=2):
+    """the denoised low-ppc run against the plain high- and low-ppc runs of
+    the same seeds: mean growth rate, peak B energy (and its standard error)
+    and GPU ms per lap of each"""
+    out = {}
+    for name, ppc, m in (("hi_plain", ppc_hi, None), ("lo_plain", ppc_lo, None), ("lo_denoised", ppc_lo, model)):
+        rs = np.array([weibel_run(2, order, n, ppc, s, m, laps=laps, ppc_norm=ppc_hi) for s in seeds])
+        out[name] = dict(growth=float(rs[:, 0].mean()), eb_max=float(rs[:, 1].mean()),
+                         eb_max_se=float(rs[:, 1].std() / np.sqrt(len(rs))), ms_per_lap=float(rs[:, 2].mean()))
+    return out
+
+
+RECIPE = dict(base_passes=8, amplitude_weight=8.0, iters=3000, lr=2e-3)
+
+
 def train(pairs, dim, iters=2000, lr=2e-3, seed=0, base_passes=0, order=2, amplitude_weight=0.0):
     """a model trained on (raw low-ppc, target) pairs; with base_passes > 0
     it learns the correction of the current after that many filter passes;
@@ -296,7 +340,11 @@ def main():
     ap.add_argument("--seeds", type=int, nargs="+", default=[0, 1])
     ap.add_argument("--test-seed", type=int, default=99)
     ap.add_argument("--iters", type=int, default=2000)
-    ap.add_argument("--base-passes", type=int, default=4)
+    ap.add_argument("--base-passes", type=int, default=RECIPE["base_passes"])
+    ap.add_argument("--amplitude-weight", type=float, default=RECIPE["amplitude_weight"])
+    ap.add_argument("--study", type=int, default=0, metavar="N",
+                    help="then judge the model on N x N cells (8 seeds): growth rate, peak B energy and GPU time "
+                         "of the denoised low-ppc run against the plain high- and low-ppc runs")
     ap.add_argument("--out", default="")
     a = ap.parse_args()
     pairs = []
@@ -305,12 +353,21 @@ def main():
                             seed=s)
     test = make_pairs(a.dim, a.order, a.nx, a.ny, a.nz, a.ppc_hi, a.ppc_lo, steps=a.steps, every=a.every,
                       seed=a.test_seed)
-    m, losses = train(pairs, a.dim, a.iters, base_passes=a.base_passes, order=a.order)
+    m, losses = train(pairs, a.dim, a.iters, base_passes=a.base_passes, order=a.order,
+                      amplitude_weight=a.amplitude_weight)
     e = errors(m, test, 4, a.dim, a.order)
     print(f"loss {losses[:50].mean():.4g} -> {losses[-50:].mean():.4g}")
     for k, d in e.items():
         print(f"held-out relative error vs filtered high-ppc current, {k:9s}: all {d['all']:.4f}  "
               f"transverse {d['T']:.4f}  longitudinal {d['L']:.4f}")
+    if a.study:
+        r = parity_study(m, n=a.study, ppc_hi=a.ppc_hi, ppc_lo=a.ppc_lo, order=a.order)
+        h = r["hi_plain"]
+        for k, d in r.items():
+            print(f"{k:12s} growth {d['growth']:.4f} ({100 * (d['growth'] / h['growth'] - 1):+.1f}%)  peak B energy "
+                  f"{d['eb_max']:.4g} ({100 * (d['eb_max'] / h['eb_max'] - 1):+.1f}% +- "
+                  f"{100 * d['eb_max_se'] / h['eb_max']:.1f})  GPU {d['ms_per_lap']:.3f} ms/lap "
+                  f"({d['ms_per_lap'] / h['ms_per_lap']:.3f} of {a.ppc_hi} ppc)")
     if a.out:
         np.savez(a.out, params=m.get_params(), scale=m.scale(), dim=a.dim, base_passes=m.base_passes())
 
