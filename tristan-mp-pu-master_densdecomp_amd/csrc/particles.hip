@@ -21,7 +21,6 @@
 // reference recomputes the old position from the new momentum
 // (x0 = x - u c/gamma), which agrees in exact arithmetic.
 #include <algorithm>
-#include <type_traits>
 #include <vector>
 
 #include "pic_push.h"
@@ -485,7 +484,7 @@ __device__ __forceinline__ void cell_sums(int c, int r0, int n, int tx, int ty, 
     }
 }
 
-template <int ORD, bool HAS_CH, bool HAS_ID>
+template <int ORD, bool HAS_CH, bool HAS_ID, bool MULTI>
 __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
 {
     extern __shared__ __align__(16) unsigned char smb[];
@@ -539,7 +538,156 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     const int lane = __lane_id();
     const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
-    // ---- phase A: push, emit, record or deposit at once
+    // ---- the records so far: crossers deposited directly, the rest
+    // counting-sorted by cell and summed per cell (phases B and C); the
+    // record space is then empty again
+    auto drain = [&]() {
+        // ---- direct deposit of the y/z crossers
+        const int nr = (int)(*nrec & 0xffffu);
+        const int nd = (int)(*nrec >> 16);
+        for (int k = tid; k < nd; k += NT) {
+            const int r = CELLS_R - 1 - k;
+            const int c = rc[r];
+            const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
+            deposit<3, ORD, LdsSink, false>(snk, li + 2, lj + 2, lk + 2, rec[r], rec[CELLS_R + r],
+                                            rec[2 * CELLS_R + r], rec[3 * CELLS_R + r], rec[4 * CELLS_R + r],
+                                            rec[5 * CELLS_R + r], rq[r]);
+        }
+
+        // ---- phase B: counting sort of the cell-summed records by cell (the
+        // crossers above run on the first waves; the scans on the last two)
+        const int wv = tid >> 6, nwv = NT >> 6, lane6 = tid & 63;
+        if (wv == nwv - 1) {
+            // exclusive scan of cnt[0..nc)
+            int carry = 0;
+            for (int c0 = 0; c0 < nc; c0 += 64) {
+                const int c = c0 + lane6;
+                const int v = c < nc ? cnt[c] : 0;
+                int x = v;
+    #pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int y = __shfl_up(x, o, 64);
+                    if (lane6 >= o) x += y;
+                }
+                if (c < nc) start[c] = carry + x - v;
+                carry += __shfl(x, 63, 64);
+            }
+        } else if (wv == nwv - 2) {
+            // histogram of the record counts (bins 0..63)
+            for (int c = lane6; c < nc; c += 64) atomicAdd(hist + min(cnt[c], 63), 1);
+        }
+        __syncthreads();
+        for (int r = tid; r < nr; r += NT) {
+            const int c = rc[r];
+            const int pos = atomicAdd(fill + c, 1);
+            idx[start[c] + pos] = (uint16_t)r;
+        }
+        if (wv == nwv - 2) {
+            // cells in descending record count: bin b starts after all larger
+            // bins; the cells a wave then sums together have similar loop lengths
+            const int h = hist[lane6];
+            int x = h;
+    #pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_down(x, o, 64);
+                if (lane6 + o < 64) x += y;
+            }
+            hist[lane6] = x - h;
+            __builtin_amdgcn_wave_barrier();
+            for (int c = lane6; c < nc; c += 64) perm[atomicAdd(hist + min(cnt[c], 63), 1)] = (uint16_t)c;
+        }
+        __syncthreads();
+
+        // ---- phase C: per-cell register sums, two threads per cell (z halves);
+        // the half is uniform per wave, so each half is its own code path
+        for (int wk = tid; wk < 2 * nc; wk += NT) {
+            const int c = perm[wk % nc];
+            if (wk < nc) cell_sums<ORD, 0>(c, start[c], cnt[c], tl.tx, tl.ty, idx, rec, rq, snk);
+            else cell_sums<ORD, 1>(c, start[c], cnt[c], tl.tx, tl.ty, idx, rec, rq, snk);
+        }
+        __syncthreads();
+        for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
+        if (tid < 64) hist[tid] = 0;
+        if (tid == 0) *nrec = 0u;
+        __syncthreads();
+    };
+
+    // ---- phase A: push, emit, record or deposit at once.  A tile with more
+    // particles than records drains the records whenever a round of NT
+    // might not fit (the test is block-uniform: read between two barriers)
+    const bool multipass = min(a.sp[0].cnt_a[t], a.sp[0].tcap) + min(a.sp[1].cnt_a[t], a.sp[1].tcap) > CELLS_R;
+    // one round of phase A: NT particles of species `sp` starting at p0;
+    // `pn`/`an` carry the prefetched particle of the next round
+    auto round = [&](const SpecArgs& sp, int n, int64_t base, int64_t slots, int p0, Part& pn, bool& an) {
+        const bool active = an;
+        Part p = pn;
+        an = p0 + NT + tid < n;
+        if (an) load_part<3, HAS_CH, HAS_ID>(sp.a, base + p0 + NT + tid, a.pk, pn);
+        int64_t td = -1;
+        float ox = 0.0f, oy = 0.0f, oz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f, q = 0.0f;
+        int li = 0, lj = 0, lk = 0;
+        bool fits = false;
+        if (active) {
+            li = p.i - i0; lj = p.j - j0; lk = p.k - k0;
+            ox = p.ox; oy = p.oy; oz = p.oz;
+            push_one<3, ORD>(src, a.quirk2, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx, dy,
+                             dz);
+            advance(p, 3, a.nx, a.ny, a.nz, dx, dy, dz);
+            q = sp.q * p.ch;
+            fits = fits4<ORD>(oy, dy) && fits4<ORD>(oz, dz);
+        }
+        // record slots, one LDS atomic per wave: cell-summed records grow
+        // from 0 up, y/z crossers from the top down.  A wave whose block
+        // would make the two meet takes its block back and deposits at
+        // once; any block reserved while such a failed one is pending
+        // fails too, so the reserved blocks stay contiguous.
+        int r = -1;
+        {
+            const uint64_t mf = __ballot(active && fits), mc = __ballot(active && !fits);
+            const unsigned inc = (unsigned)__popcll(mf) + ((unsigned)__popcll(mc) << 16);
+            unsigned old = 0;
+            if (inc && lane == 0) old = atomicAdd(nrec, inc);
+            old = __shfl(old, 0, 64);
+            const int lo = (int)(old & 0xffffu) + __popcll(mf), hi = (int)(old >> 16) + __popcll(mc);
+            const bool ok = lo + hi <= CELLS_R;
+            if (inc && !ok && lane == 0) atomicSub(nrec, inc);
+            if (ok && active)
+                r = fits ? (int)(old & 0xffffu) + __popcll(mf & ltmask)
+                         : CELLS_R - 1 - ((int)(old >> 16) + __popcll(mc & ltmask));
+        }
+        if (active) {
+            const int c = li + tl.tx * (lj + tl.ty * lk);
+            if (r >= 0) {
+                rec[0 * CELLS_R + r] = ox;
+                rec[1 * CELLS_R + r] = oy;
+                rec[2 * CELLS_R + r] = oz;
+                rec[3 * CELLS_R + r] = dx;
+                rec[4 * CELLS_R + r] = dy;
+                rec[5 * CELLS_R + r] = dz;
+                rq[r] = q;
+                rc[r] = (uint16_t)c;
+                if (fits) atomicAdd(cnt + c, 1);
+            } else {
+                // record space exhausted: deposit this particle at once
+                deposit<3, ORD>(snk, li + 2, lj + 2, lk + 2, ox, oy, oz, dx, dy, dz, q);
+            }
+            td = tile_of(tl, p.i, p.j, p.k);
+        }
+        // slots in b: leavers reserve one each in their new tile, the
+        // wave's stayers one block in this tile; both atomics are in
+        // flight together (one round trip)
+        const bool stay = active && td == t;
+        const uint64_t m = __ballot(stay);
+        int32_t lslot = 0, b0 = 0;
+        if (active && !stay) lslot = atomicAdd(sp.cnt_b + td, 1);
+        const int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
+        if (m && lane == leader) b0 = atomicAdd(sp.cnt_b + t, __popcll(m));
+        b0 = __shfl(b0, leader, 64);
+        const int32_t slot = stay ? b0 + __popcll(m & ltmask) : lslot;
+        if (active) emit<3, HAS_CH, HAS_ID>(sp, a.pk, slots, td, slot, p);
+    };
+
+    if constexpr (!MULTI) {
 #pragma unroll 1
     for (int s = 0; s < 2; ++s) {
         const SpecArgs& sp = a.sp[s];
@@ -550,141 +698,45 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         Part pn;
         bool an = tid < n;
         if (an) load_part<3, HAS_CH, HAS_ID>(sp.a, base + tid, a.pk, pn);
-        for (int p0 = 0; p0 < n; p0 += NT) {
-            const bool active = an;
-            Part p = pn;
-            an = p0 + NT + tid < n;
-            if (an) load_part<3, HAS_CH, HAS_ID>(sp.a, base + p0 + NT + tid, a.pk, pn);
-            int64_t td = -1;
-            float ox = 0.0f, oy = 0.0f, oz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f, q = 0.0f;
-            int li = 0, lj = 0, lk = 0;
-            bool fits = false;
-            if (active) {
-                li = p.i - i0; lj = p.j - j0; lk = p.k - k0;
-                ox = p.ox; oy = p.oy; oz = p.oz;
-                push_one<3, ORD>(src, a.quirk2, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx, dy,
-                                 dz);
-                advance(p, 3, a.nx, a.ny, a.nz, dx, dy, dz);
-                q = sp.q * p.ch;
-                fits = fits4<ORD>(oy, dy) && fits4<ORD>(oz, dz);
+        for (int p0 = 0; p0 < n; p0 += NT) round(sp, n, base, slots, p0, pn, an);
+    }
+    __syncthreads();
+    drain();
+    } else {
+    // dense tiles: phase A stops when a round might not fit, the records
+    // are drained, and phase A resumes at the block-uniform point
+    // (species s, round p0); the test reads the record count between two
+    // barriers
+    int s = 0, p0 = 0;
+#pragma unroll 1
+    for (;;) {
+#pragma unroll 1
+    for (; s < 2; ++s, p0 = 0) {
+        const SpecArgs& sp = a.sp[s];
+        const int n = min(sp.cnt_a[t], sp.tcap);
+        const int64_t base = t * sp.tcap;
+        const int64_t slots = (int64_t)tl.ntiles * sp.tcap;
+        Part pn;
+        bool an = p0 + tid < n;
+        if (an) load_part<3, HAS_CH, HAS_ID>(sp.a, base + p0 + tid, a.pk, pn);
+        bool full = false;
+        for (; p0 < n; p0 += NT) {
+            if (multipass) {
+                __syncthreads();
+                const unsigned used = *nrec;
+                __syncthreads();
+                full = (int)(used & 0xffffu) + (int)(used >> 16) + NT > CELLS_R;
+                if (full) break;
             }
-            // record slots, one LDS atomic per wave: cell-summed records grow
-            // from 0 up, y/z crossers from the top down.  A wave whose block
-            // would make the two meet takes its block back and deposits at
-            // once; any block reserved while such a failed one is pending
-            // fails too, so the reserved blocks stay contiguous.
-            int r = -1;
-            {
-                const uint64_t mf = __ballot(active && fits), mc = __ballot(active && !fits);
-                const unsigned inc = (unsigned)__popcll(mf) + ((unsigned)__popcll(mc) << 16);
-                unsigned old = 0;
-                if (inc && lane == 0) old = atomicAdd(nrec, inc);
-                old = __shfl(old, 0, 64);
-                const int lo = (int)(old & 0xffffu) + __popcll(mf), hi = (int)(old >> 16) + __popcll(mc);
-                const bool ok = lo + hi <= CELLS_R;
-                if (inc && !ok && lane == 0) atomicSub(nrec, inc);
-                if (ok && active)
-                    r = fits ? (int)(old & 0xffffu) + __popcll(mf & ltmask)
-                             : CELLS_R - 1 - ((int)(old >> 16) + __popcll(mc & ltmask));
-            }
-            if (active) {
-                const int c = li + tl.tx * (lj + tl.ty * lk);
-                if (r >= 0) {
-                    rec[0 * CELLS_R + r] = ox;
-                    rec[1 * CELLS_R + r] = oy;
-                    rec[2 * CELLS_R + r] = oz;
-                    rec[3 * CELLS_R + r] = dx;
-                    rec[4 * CELLS_R + r] = dy;
-                    rec[5 * CELLS_R + r] = dz;
-                    rq[r] = q;
-                    rc[r] = (uint16_t)c;
-                    if (fits) atomicAdd(cnt + c, 1);
-                } else {
-                    // record space exhausted: deposit this particle at once
-                    deposit<3, ORD>(snk, li + 2, lj + 2, lk + 2, ox, oy, oz, dx, dy, dz, q);
-                }
-                td = tile_of(tl, p.i, p.j, p.k);
-            }
-            // slots in b: leavers reserve one each in their new tile, the
-            // wave's stayers one block in this tile; both atomics are in
-            // flight together (one round trip)
-            const bool stay = active && td == t;
-            const uint64_t m = __ballot(stay);
-            int32_t lslot = 0, b0 = 0;
-            if (active && !stay) lslot = atomicAdd(sp.cnt_b + td, 1);
-            const int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
-            if (m && lane == leader) b0 = atomicAdd(sp.cnt_b + t, __popcll(m));
-            b0 = __shfl(b0, leader, 64);
-            const int32_t slot = stay ? b0 + __popcll(m & ltmask) : lslot;
-            if (active) emit<3, HAS_CH, HAS_ID>(sp, a.pk, slots, td, slot, p);
+            round(sp, n, base, slots, p0, pn, an);
         }
+        if (full) break;
     }
     __syncthreads();
-
-    // ---- direct deposit of the y/z crossers
-    const int nr = (int)(*nrec & 0xffffu);
-    const int nd = (int)(*nrec >> 16);
-    for (int k = tid; k < nd; k += NT) {
-        const int r = CELLS_R - 1 - k;
-        const int c = rc[r];
-        const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
-        deposit<3, ORD, LdsSink, false>(snk, li + 2, lj + 2, lk + 2, rec[r], rec[CELLS_R + r],
-                                        rec[2 * CELLS_R + r], rec[3 * CELLS_R + r], rec[4 * CELLS_R + r],
-                                        rec[5 * CELLS_R + r], rq[r]);
+    drain();
+    if (s >= 2) break;
     }
-
-    // ---- phase B: counting sort of the cell-summed records by cell (the
-    // crossers above run on the first waves; the scans on the last two)
-    const int wv = tid >> 6, nwv = NT >> 6, lane6 = tid & 63;
-    if (wv == nwv - 1) {
-        // exclusive scan of cnt[0..nc)
-        int carry = 0;
-        for (int c0 = 0; c0 < nc; c0 += 64) {
-            const int c = c0 + lane6;
-            const int v = c < nc ? cnt[c] : 0;
-            int x = v;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(x, o, 64);
-                if (lane6 >= o) x += y;
-            }
-            if (c < nc) start[c] = carry + x - v;
-            carry += __shfl(x, 63, 64);
-        }
-    } else if (wv == nwv - 2) {
-        // histogram of the record counts (bins 0..63)
-        for (int c = lane6; c < nc; c += 64) atomicAdd(hist + min(cnt[c], 63), 1);
     }
-    __syncthreads();
-    for (int r = tid; r < nr; r += NT) {
-        const int c = rc[r];
-        const int pos = atomicAdd(fill + c, 1);
-        idx[start[c] + pos] = (uint16_t)r;
-    }
-    if (wv == nwv - 2) {
-        // cells in descending record count: bin b starts after all larger
-        // bins; the cells a wave then sums together have similar loop lengths
-        const int h = hist[lane6];
-        int x = h;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_down(x, o, 64);
-            if (lane6 + o < 64) x += y;
-        }
-        hist[lane6] = x - h;
-        __builtin_amdgcn_wave_barrier();
-        for (int c = lane6; c < nc; c += 64) perm[atomicAdd(hist + min(cnt[c], 63), 1)] = (uint16_t)c;
-    }
-    __syncthreads();
-
-    // ---- phase C: per-cell register sums, two threads per cell (z halves);
-    // the half is uniform per wave, so each half is its own code path
-    for (int wk = tid; wk < 2 * nc; wk += NT) {
-        const int c = perm[wk % nc];
-        if (wk < nc) cell_sums<ORD, 0>(c, start[c], cnt[c], tl.tx, tl.ty, idx, rec, rq, snk);
-        else cell_sums<ORD, 1>(c, start[c], cnt[c], tl.tx, tl.ty, idx, rec, rq, snk);
-    }
-    __syncthreads();
     for (int e = tid; e < jplane; e += NT) {
         const int li = e % a.JX;
         const int r = e / a.JX;
@@ -1093,7 +1145,7 @@ int particles_bin(tmpg_ctx* ctx, int s, const PBuf& flat, int64_t n, bool has_ch
     if (tcap > (int64_t)1 << 30) { ctx->err = "tile capacity too large"; return TMPG_EINVAL; }
     sp.tcap = (int32_t)tcap;
     sp.slots = tl.ntiles * tcap;
-    sp.ocap = std::max<int64_t>(65536, n / 64);
+    sp.ocap = std::max<int64_t>(65536, n / 16);
     int rc = 0;
     rc |= alloc_pbuf(ctx, sp.a, sp.slots, has_ch, has_id);
     rc |= alloc_pbuf(ctx, sp.b, sp.slots, has_ch, has_id);
@@ -1286,12 +1338,21 @@ int particles_push(tmpg_ctx* ctx, int mode)
         const CellsLds CL = cells_lds(a.FX, a.FY, a.FZ, a.JX, a.JY, a.JZ, ncell_tile);
         const bool cells = g.dim == 3 && ctx->use_cell_sums && ncell_tile <= CELLS_T / 2 * 2 &&
                            CL.bytes <= 160 * 1024;
+        // order 3 only: there a direct deposit (about 300 LDS atomics per
+        // particle) costs more than draining the records in passes; at
+        // order 2 the direct deposits of the excess particles are cheaper
+        const bool dense = ctx->cfg.order == 3 &&
+                           ctx->sp[0].n + ctx->sp[1].n > (int64_t)CELLS_R * 9 / 10 * tl.ntiles;
         if (ctx->push_events) (void)hipEventRecord(ctx->push_events[0], ctx->stream);
         dispatch(g.dim, ctx->cfg.order, ch, id, [&](auto D, auto O, auto C, auto I) {
             constexpr int Dv = decltype(D)::value, Ov = decltype(O)::value;
             constexpr bool Cv = decltype(C)::value, Iv = decltype(I)::value;
             if (Dv == 3 && cells) {
-                auto k = k_push_cells<Ov, Cv, Iv>;
+                // tiles that hold more particles than deposit records on
+                // average get the kernel that drains its records in passes
+                auto k = k_push_cells<Ov, Cv, Iv, false>;
+                if constexpr (Ov == 3)
+                    if (dense) k = k_push_cells<Ov, Cv, Iv, true>;
                 (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)CL.bytes);
                 hipLaunchKernelGGL(k, dim3((unsigned)tl.ntiles), dim3(CELLS_T), CL.bytes, ctx->stream, a, CL);
             } else {

@@ -243,6 +243,8 @@ int tmpg_destroy(tmpg_ctx* ctx)
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : ctx->prof) (void)hipEventDestroy(ev);
+    if (ctx->cnt_ev) (void)hipEventDestroy(ctx->cnt_ev);
+    if (ctx->hostcnt) (void)hipHostFree(ctx->hostcnt);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TMPG_OK;
@@ -512,7 +514,30 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
 {
     if (!valid_ctx(ctx) || nsteps < 0) return TMPG_EINVAL;
     int rc = 0;
+    if (!ctx->hostcnt) {
+        TMPG_HIP_CHECK(ctx, hipHostMalloc((void**)&ctx->hostcnt, 8 * sizeof(int32_t), hipHostMallocDefault));
+        TMPG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->cnt_ev, hipEventDisableTiming));
+    }
     for (int s = 0; s < nsteps && rc == 0; ++s) {
+        if (ctx->cnt_pending) {
+            // overflow counters copied at an earlier step, read without a
+            // wait: a filling overflow list re-bins now rather than at the
+            // next 32-step check
+            if (hipEventQuery(ctx->cnt_ev) == hipSuccess) {
+                ctx->cnt_pending = false;
+                bool check = false;
+                for (int q = 0; q < 2; ++q)
+                    check |= ctx->sp[q].ocnt && (ctx->hostcnt[4 * q + 2] > 0 ||
+                                                 (int64_t)ctx->hostcnt[4 * q] > ctx->sp[q].ocap / 8);
+                if (check) {
+                    ctx->steps_since_check = 0;
+                    rc = particles_check(ctx);
+                    if (rc) break;
+                }
+            } else {
+                (void)hipGetLastError();   // not ready yet: no sticky status
+            }
+        }
         if (++ctx->steps_since_check >= 32) {
             // lost particles are an error; a filling overflow list re-bins
             ctx->steps_since_check = 0;
@@ -550,6 +575,15 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
         (void)hipEventRecord(ev[4], ctx->stream);
         if (!rc) rc = fields_efull(ctx, true);
         (void)hipEventRecord(ev[5], ctx->stream);
+        if (!rc && !ctx->cnt_pending && (ctx->steps_since_check & 3) == 0) {
+            for (int q = 0; q < 2; ++q) {
+                if (ctx->sp[q].ocnt)
+                    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(ctx->hostcnt + 4 * q, ctx->sp[q].ocnt, 4 * sizeof(int32_t),
+                                                       hipMemcpyDeviceToHost, ctx->stream));
+            }
+            TMPG_HIP_CHECK(ctx, hipEventRecord(ctx->cnt_ev, ctx->stream));
+            ctx->cnt_pending = true;
+        }
     }
     ctx->move_pending = false;
     return rc;

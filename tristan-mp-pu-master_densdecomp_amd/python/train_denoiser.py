@@ -277,12 +277,12 @@ def compare_runs(model, dim=2, order=2, nx=96, ny=96, nz=1, ppc_hi=32, ppc_lo=4,
 
 def weibel_run(dim, order, n, ppc, seed, model=None, laps=600, every=10, ppc_norm=32, gb=0.5, theta=2.5e-3,
                c_omp=5.0, ntimes=4):
-    """one 2D pair Weibel run on n x n cells at `ppc` particles per cell per
+    """one pair Weibel run on n^dim cells at `ppc` particles per cell per
     species (charge and mass scaled to the density of `ppc_norm`), with the
     denoiser in place of the filter when `model` is given.  Returns the
     B-energy growth rate, the peak B energy and the GPU time per lap."""
-    ctx, _ = make_weibel(dim, order, n, n, 1, ppc_norm, gb, theta, seed, c_omp=c_omp, ntimes=ntimes,
-                         q_scale=ppc_norm / ppc, load=False)
+    ctx, _ = make_weibel(dim, order, n, n, n if dim == 3 else 1, ppc_norm, gb, theta, seed, c_omp=c_omp,
+                         ntimes=ntimes, q_scale=ppc_norm / ppc, load=False)
     for k, (s, d) in enumerate(((0, -gb), (1, -gb), (0, gb), (1, gb))):
         ctx.load_uniform(s, ppc // 2, d, theta, 4 * seed + k + 1)
     if model is not None:
@@ -297,13 +297,19 @@ def weibel_run(dim, order, n, ppc, seed, model=None, laps=600, every=10, ppc_nor
     return growth_rate(laps_, eb, lo_frac=1e-2, hi_frac=0.3), float(eb.max()), float(ms[4] / max(ms[3], 1))
 
 
-def parity_study(model, n=192, seeds=range(11, 19), ppc_hi=32, ppc_lo=2, laps=600, order=2):
+def parity_study(model, n=192, seeds=range(11, 19), ppc_hi=32, ppc_lo=2, laps=600, order=2, dim=2, c_omp=5.0,
+                 log=None):
     """the denoised low-ppc run against the plain high- and low-ppc runs of
     the same seeds: mean growth rate, peak B energy (and its standard error)
     and GPU ms per lap of each"""
     out = {}
     for name, ppc, m in (("hi_plain", ppc_hi, None), ("lo_plain", ppc_lo, None), ("lo_denoised", ppc_lo, model)):
-        rs = np.array([weibel_run(2, order, n, ppc, s, m, laps=laps, ppc_norm=ppc_hi) for s in seeds])
+        rs = []
+        for s in seeds:
+            rs.append(weibel_run(dim, order, n, ppc, s, m, laps=laps, ppc_norm=ppc_hi, c_omp=c_omp))
+            if log:
+                log(f"{name} seed {s}: growth {rs[-1][0]:.4f} peak B {rs[-1][1]:.4g} {rs[-1][2]:.3f} ms/lap")
+        rs = np.array(rs)
         out[name] = dict(growth=float(rs[:, 0].mean()), eb_max=float(rs[:, 1].mean()),
                          eb_max_se=float(rs[:, 1].std() / np.sqrt(len(rs))), ms_per_lap=float(rs[:, 2].mean()))
     return out
