@@ -60,7 +60,10 @@ typedef struct tmpg_config {
     int32_t tile[3];      /* cell tile for the sorted push (0: default) */
     int32_t sort_every;   /* tmpg_step sorts each tile's particles by cell every
                              this many steps (0: every 4, negative: never) */
-    int32_t reserved[4];
+    int32_t highorder;    /* nonzero: 4th-order field solver (<algorithm>
+                             highorder; advance_b_halfstep_42 and
+                             advance_e_fullstep_42, code/fields.F90:1039-1361) */
+    int32_t reserved[3];
 } tmpg_config;
 
 typedef struct tmpg_ctx tmpg_ctx;

@@ -36,7 +36,7 @@ CASES = {
 PHYS = dict(c=0.45, corr=1.025, ntimes=4, c_omp=3.0, gamma0=0.5, ppc0=2.0, me=1.0, mi=4.0)
 
 
-def input_text(dim, nx, ny, nz, maxptl):
+def input_text(dim, nx, ny, nz, maxptl, highorder=0):
     p = PHYS
     return f"""<node_configuration>
 sizex = 1
@@ -49,7 +49,7 @@ mx0 = {nx}
 my0 = {ny}
 mz0 = {nz if dim == 3 else 1}
 <algorithm>
-highorder = 0
+highorder = {highorder}
 Corr = {p['corr']}
 ntimes = {p['ntimes']}
 <boundaries>
@@ -187,5 +187,47 @@ def main():
             print(out, os.path.getsize(out))
 
 
+def main_highorder():
+    """ref_{dim}d_ord{order}_ho.npz: the same kind of state run with
+    highorder = 1 (the 4th-order field solver, advance_b_halfstep_42 and
+    advance_e_fullstep_42): one full main-loop step, and the Yee part alone
+    (B half, B half, E; driver mode 4)"""
+    for dim in (2, 3):
+        for order in (1, 3):
+            case = CASES[dim]
+            cfg = orc.make_cfg(dim, order, case["nx"], case["ny"], case["nz"], ntimes=PHYS["ntimes"])
+            rng = np.random.default_rng(5000 + 1000 * dim + order)
+            nz = case["nz"] if dim == 3 else 1
+            f_in = np.empty((6, nz, case["ny"], case["nx"]), dtype=np.float32)
+            f_in[:3] = rng.uniform(-0.05, 0.05, f_in[:3].shape)
+            f_in[3:] = rng.uniform(-0.1, 0.1, f_in[3:].shape)
+            pi = make_particles(rng, cfg, case["npart"])
+            pe = make_particles(rng, cfg, case["npart"] + 17)
+            maxptl = 20000
+            maxhlf = maxptl // 2
+            inp = input_text(dim, case["nx"], case["ny"], case["nz"], maxptl, highorder=1)
+            fix = dict(f_in=f_in, pi_in=pi, pe_in=pe)
+            sl = orc.interior_slices(cfg)
+            with tempfile.TemporaryDirectory() as td:
+                for prec, dt in (("f32", np.float32), ("f64", np.float64)):
+                    f6 = orc.fill_from_interior(cfg, f_in.astype(dt), dt)
+                    st = os.path.join(td, f"state_{prec}.bin")
+                    write_state(st, f6, pi, pe, dt)
+                    g, qi_, qe_ = run_ref(dim, order, prec, inp, st, 0, cfg, pi.shape[1], pe.shape[1], maxhlf)
+                    fix[f"step_{prec}"] = g[(slice(None),) + sl]
+                    fix[f"step_pi_{prec}"] = qi_
+                    fix[f"step_pe_{prec}"] = qe_
+                    g, _, _ = run_ref(dim, order, prec, inp, st, 4, cfg, pi.shape[1], pe.shape[1], maxhlf)
+                    fix[f"yee_{prec}"] = g[(slice(0, 6),) + sl]
+            params = dict(dim=dim, order=order, nx=case["nx"], ny=case["ny"], nz=nz, highorder=1, **PHYS)
+            fix["params"] = np.array(repr(params))
+            out = os.path.join(HERE, f"ref_{dim}d_ord{order}_ho.npz")
+            np.savez_compressed(out, **fix)
+            print(out, os.path.getsize(out))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["highorder"]:
+        main_highorder()
+    else:
+        main()

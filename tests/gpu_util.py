@@ -15,8 +15,8 @@ def rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
-def load_fixture(dim, order):
-    fx = np.load(os.path.join(GOLD, f"ref_{dim}d_ord{order}.npz"))
+def load_fixture(dim, order, suffix=""):
+    fx = np.load(os.path.join(GOLD, f"ref_{dim}d_ord{order}{suffix}.npz"))
     p = ast.literal_eval(str(fx["params"]))
     return fx, p
 
@@ -30,7 +30,7 @@ def norm64(p):
 
 
 def make_context(tmpg, dim, order, nx, ny, nz, p, ntimes=4, compat=None, track_ids=True, capacity=(0, 0),
-                 sort_every=0):
+                 sort_every=0, highorder=0):
     nm = norm32(p)
     kw = {}
     if compat is not None:
@@ -38,7 +38,7 @@ def make_context(tmpg, dim, order, nx, ny, nz, p, ntimes=4, compat=None, track_i
     return tmpg.Context(dim, order, nx, ny, nz, ntimes=ntimes, c=p["c"], corr=p["corr"],
                         qm=(float(nm["qmi"]), float(nm["qme"])), q=(float(nm["qi"]), float(nm["qe"])),
                         mass=(float(nm["mi"]), float(nm["me"])), track_ids=track_ids, capacity=capacity,
-                        sort_every=sort_every, **kw)
+                        sort_every=sort_every, highorder=highorder, **kw)
 
 
 def sorted_by_id(prt, ids):

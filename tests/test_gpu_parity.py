@@ -74,6 +74,50 @@ def test_step_vs_reference_fp64(tmpg, dim, order):
         _check_particles(ctx, got, ids, fx[key], f"species {s}")
 
 
+HO_CASES = [(2, 1), (2, 3), (3, 1), (3, 3)]
+
+
+@pytest.mark.parametrize("dim,order", HO_CASES)
+def test_highorder_yee_vs_reference_fp64(tmpg, dim, order):
+    """the 4th-order field solver (highorder = 1): B half, B half, E against
+    the compiled reference's FP64 build (tests/golden/ref_*_ho.npz)"""
+    fx, p = load_fixture(dim, order, "_ho")
+    ctx = make_context(tmpg, dim, order, p["nx"], p["ny"], p["nz"], p, ntimes=p["ntimes"], highorder=1)
+    cfg = orc.make_cfg(dim, order, p["nx"], p["ny"], p["nz"])
+    ctx.set_fields(orc.fill_from_interior(cfg, fx["f_in"], np.float32)[:6])
+    ctx.advance_b_half()
+    ctx.advance_b_half()
+    ctx.advance_e()
+    f = _interior(ctx, ctx.get_fields())
+    ref = fx["yee_f64"]
+    e_e, e_b = rel(f[:3], ref[:3]), rel(f[3:6], ref[3:6])
+    print(f"highorder dim{dim} ord{order}: E {e_e:.2e} B {e_b:.2e}")
+    assert e_e < TOL_FIELD and e_b < TOL_FIELD
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,order", HO_CASES)
+def test_highorder_step_vs_reference_fp64(tmpg, dim, order):
+    """one main-loop step with the 4th-order field solver"""
+    fx, p = load_fixture(dim, order, "_ho")
+    ctx = make_context(tmpg, dim, order, p["nx"], p["ny"], p["nz"], p, ntimes=p["ntimes"], highorder=1)
+    cfg = orc.make_cfg(dim, order, p["nx"], p["ny"], p["nz"])
+    ctx.set_fields(orc.fill_from_interior(cfg, fx["f_in"], np.float32)[:6])
+    ctx.set_particles(0, fx["pi_in"])
+    ctx.set_particles(1, fx["pe_in"])
+    ctx.step(1)
+    f = _interior(ctx, ctx.get_fields())
+    j = _interior(ctx, ctx.get_currents())
+    ref = fx["step_f64"]
+    e_e, e_b, e_j = rel(f[:3], ref[:3]), rel(f[3:6], ref[3:6]), rel(j, ref[6:9])
+    print(f"highorder dim{dim} ord{order}: E {e_e:.2e} B {e_b:.2e} J {e_j:.2e}")
+    assert e_e < TOL_FIELD and e_b < TOL_FIELD and e_j < TOL_FIELD
+    for s, key in ((0, "step_pi_f64"), (1, "step_pe_f64")):
+        got, ids = ctx.get_particles(s, with_ids=True)
+        _check_particles(ctx, got, ids, fx[key], f"species {s}")
+    ctx.close()
+
+
 @pytest.mark.parametrize("dim,order", CASES)
 def test_push_deposit_vs_oracle_fp64(tmpg, dim, order):
     """fused push+deposit (no filter) on a larger random state"""

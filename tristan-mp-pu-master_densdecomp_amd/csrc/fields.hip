@@ -5,6 +5,9 @@
 // E step     : code/fields.F90:739-870 (advance_e_fullstep), optionally fused
 //              with add_current (code/fields.F90:1372-1395): E = (E + c*curl B) + J,
 //              the reference's two roundings in its order
+// highorder  : code/fields.F90:1039-1361 (advance_b_halfstep_42,
+//              advance_e_fullstep_42): differences over 1 and 3 cells,
+//              weights 9/8 and -1/24
 // filter     : code/filter.F90:8-221, (1,2,1)^d / 4^d per pass, periodic;
 //              up to four passes per sweep as the separable binomial
 // The reference fills ghost layers (bc_b1/bc_e1) and folds ghost currents;
@@ -223,6 +226,76 @@ __global__ void k_zero(float* __restrict__ a, int64_t n)
     for (; t < n; t += stride) a[t] = 0.0f;
 }
 
+// 4th-order curl: d f / d x at the half point = (9/8)(f[+1] - f[0]) - (1/24)(f[+2] - f[-1])
+// (forward for B, backward for E), summed in the reference's order:
+// b + c1*(pair of 1-cell differences) + c2*(pair of 3-cell differences)
+template <int DIM>
+__global__ void __launch_bounds__(256) k_bhalf42(const float* __restrict__ ex, const float* __restrict__ ey,
+                                                 const float* __restrict__ ez, float* __restrict__ bx,
+                                                 float* __restrict__ by, float* __restrict__ bz, int nx, int ny,
+                                                 int nz, float c1, float c2)
+{
+    const int i = blockIdx.x * BX_ + threadIdx.x;
+    const int j = blockIdx.y * BY_ + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    auto at = [&](int ii, int jj, int kk) {
+        return wrapm(ii, nx) + (int64_t)nx * (wrapm(jj, ny) + (int64_t)ny * (DIM == 3 ? wrapm(kk, nz) : 0));
+    };
+    const int64_t o = at(i, j, k);
+    const int64_t ip1 = at(i + 1, j, k), ip2 = at(i + 2, j, k), im1 = at(i - 1, j, k);
+    const int64_t jp1 = at(i, j + 1, k), jp2 = at(i, j + 2, k), jm1 = at(i, j - 1, k);
+    if (DIM == 3) {
+        const int64_t kp1 = at(i, j, k + 1), kp2 = at(i, j, k + 2), km1 = at(i, j, k - 1);
+        bx[o] = bx[o] + c1 * (ey[kp1] - ey[o] - ez[jp1] + ez[o]) + c2 * (ey[kp2] - ey[km1] - ez[jp2] + ez[jm1]);
+        by[o] = by[o] + c1 * (ez[ip1] - ez[o] - ex[kp1] + ex[o]) + c2 * (ez[ip2] - ez[im1] - ex[kp2] + ex[km1]);
+        bz[o] = bz[o] + c1 * (ex[jp1] - ex[o] - ey[ip1] + ey[o]) + c2 * (ex[jp2] - ex[jm1] - ey[ip2] + ey[im1]);
+    } else {
+        bx[o] = bx[o] + c1 * (-ez[jp1] + ez[o]) + c2 * (-ez[jp2] + ez[jm1]);
+        by[o] = by[o] + c1 * (ez[ip1] - ez[o]) + c2 * (ez[ip2] - ez[im1]);
+        bz[o] = bz[o] + c1 * (ex[jp1] - ex[o] - ey[ip1] + ey[o]) + c2 * (ex[jp2] - ex[jm1] - ey[ip2] + ey[im1]);
+    }
+}
+
+template <int DIM, bool ADDJ>
+__global__ void __launch_bounds__(256) k_efull42(float* __restrict__ ex, float* __restrict__ ey,
+                                                 float* __restrict__ ez, const float* __restrict__ bx,
+                                                 const float* __restrict__ by, const float* __restrict__ bz,
+                                                 const float* __restrict__ jx, const float* __restrict__ jy,
+                                                 const float* __restrict__ jz, int nx, int ny, int nz, float c1,
+                                                 float c2)
+{
+    const int i = blockIdx.x * BX_ + threadIdx.x;
+    const int j = blockIdx.y * BY_ + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    auto at = [&](int ii, int jj, int kk) {
+        return wrapm(ii, nx) + (int64_t)nx * (wrapm(jj, ny) + (int64_t)ny * (DIM == 3 ? wrapm(kk, nz) : 0));
+    };
+    const int64_t o = at(i, j, k);
+    const int64_t im1 = at(i - 1, j, k), im2 = at(i - 2, j, k), ip1 = at(i + 1, j, k);
+    const int64_t jm1 = at(i, j - 1, k), jm2 = at(i, j - 2, k), jp1 = at(i, j + 1, k);
+    float nex, ney, nez;
+    if (DIM == 3) {
+        const int64_t km1 = at(i, j, k - 1), km2 = at(i, j, k - 2), kp1 = at(i, j, k + 1);
+        nex = ex[o] + c1 * (by[km1] - by[o] - bz[jm1] + bz[o]) + c2 * (by[km2] - by[kp1] - bz[jm2] + bz[jp1]);
+        ney = ey[o] + c1 * (bz[im1] - bz[o] - bx[km1] + bx[o]) + c2 * (bz[im2] - bz[ip1] - bx[km2] + bx[kp1]);
+        nez = ez[o] + c1 * (bx[jm1] - bx[o] - by[im1] + by[o]) + c2 * (bx[jm2] - bx[jp1] - by[im2] + by[ip1]);
+    } else {
+        nex = ex[o] + c1 * (-bz[jm1] + bz[o]) + c2 * (-bz[jm2] + bz[jp1]);
+        ney = ey[o] + c1 * (bz[im1] - bz[o]) + c2 * (bz[im2] - bz[ip1]);
+        nez = ez[o] + c1 * (bx[jm1] - bx[o] - by[im1] + by[o]) + c2 * (bx[jm2] - bx[jp1] - by[im2] + by[ip1]);
+    }
+    if (ADDJ) {
+        nex = nex + jx[o];
+        ney = ney + jy[o];
+        nez = nez + jz[o];
+    }
+    ex[o] = nex;
+    ey[o] = ney;
+    ez[o] = nez;
+}
+
 dim3 grid_of(const Grid& g) { return dim3((g.nx + BX_ - 1) / BX_, (g.ny + BY_ - 1) / BY_, g.nz); }
 
 }  // namespace
@@ -232,6 +305,19 @@ int fields_bhalf(tmpg_ctx* ctx)
     const Grid& g = ctx->g;
     float cst = ctx->cfg.corr * (0.5f * ctx->cfg.c);
     float** f = ctx->f;
+    if (ctx->cfg.highorder) {
+        // code/fields.F90:1050-1052: coef1 = 9/8*corr*(c/2), coef2 = -1/24*corr*(c/2)
+        const float c1 = 9.0f / 8.0f * ctx->cfg.corr * (0.5f * ctx->cfg.c);
+        const float c2 = -1.0f / 24.0f * ctx->cfg.corr * (0.5f * ctx->cfg.c);
+        if (g.dim == 3)
+            hipLaunchKernelGGL(k_bhalf42<3>, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, f[EX], f[EY], f[EZ],
+                               f[BX], f[BY], f[BZ], g.nx, g.ny, g.nz, c1, c2);
+        else
+            hipLaunchKernelGGL(k_bhalf42<2>, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, f[EX], f[EY], f[EZ],
+                               f[BX], f[BY], f[BZ], g.nx, g.ny, g.nz, c1, c2);
+        TMPG_HIP_CHECK(ctx, hipGetLastError());
+        return TMPG_OK;
+    }
     if (g.dim == 3)
         hipLaunchKernelGGL(k_bhalf<3>, grid_of(g), dim3(BX_, BY_), 0, ctx->stream, f[EX], f[EY], f[EZ], f[BX],
                            f[BY], f[BZ], g.nx, g.ny, g.nz, cst);
@@ -247,6 +333,19 @@ int fields_efull(tmpg_ctx* ctx, bool add_j)
     const Grid& g = ctx->g;
     float cst = ctx->cfg.corr * ctx->cfg.c;
     float** f = ctx->f;
+    if (ctx->cfg.highorder) {
+        // code/fields.F90:1234-1236: coef1 = 9/8*corr*c, coef2 = -1/24*corr*c
+        const float c1 = 9.0f / 8.0f * ctx->cfg.corr * ctx->cfg.c;
+        const float c2 = -1.0f / 24.0f * ctx->cfg.corr * ctx->cfg.c;
+#define TMPG_EF42(D, A)                                                                                   \
+    hipLaunchKernelGGL((k_efull42<D, A>), grid_of(g), dim3(BX_, BY_), 0, ctx->stream, f[EX], f[EY], f[EZ], \
+                       f[BX], f[BY], f[BZ], f[JX], f[JY], f[JZ], g.nx, g.ny, g.nz, c1, c2)
+        if (g.dim == 3) { if (add_j) TMPG_EF42(3, true); else TMPG_EF42(3, false); }
+        else { if (add_j) TMPG_EF42(2, true); else TMPG_EF42(2, false); }
+#undef TMPG_EF42
+        TMPG_HIP_CHECK(ctx, hipGetLastError());
+        return TMPG_OK;
+    }
 #define TMPG_EF(D, A)                                                                                   \
     hipLaunchKernelGGL((k_efull<D, A>), grid_of(g), dim3(BX_, BY_), 0, ctx->stream, f[EX], f[EY], f[EZ], \
                        f[BX], f[BY], f[BZ], f[JX], f[JY], f[JZ], g.nx, g.ny, g.nz, cst)

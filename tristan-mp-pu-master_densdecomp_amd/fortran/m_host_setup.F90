@@ -63,10 +63,6 @@ contains
 		call gi("gpu", "seed", 0, hp%seed)
 		call tmpg_check(tmpg_input_close(inp), "tmpg_input_close", c_null_ptr)
 		if (hp%dim == 2) hp%nz = 1
-		if (hp%highorder /= 0) then
-			write(*,*) "highorder=1 (4th-order field solver) is not on this path; use highorder=0"
-			stop 2
-		endif
 		call normalise(hp)
 	contains
 		subroutine gi(b, k, d, v)
@@ -121,6 +117,7 @@ contains
 		cfg%capacity = 0
 		cfg%tile = 0
 		cfg%sort_every = 0
+		cfg%highorder = merge(1, 0, hp%highorder /= 0)
 		cfg%reserved = 0
 	end subroutine make_config
 

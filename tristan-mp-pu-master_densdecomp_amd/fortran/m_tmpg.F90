@@ -20,7 +20,8 @@ module m_tmpg
 		integer(c_int64_t) :: capacity(2)
 		integer(c_int32_t) :: tile(3)
 		integer(c_int32_t) :: sort_every
-		integer(c_int32_t) :: reserved(4)
+		integer(c_int32_t) :: highorder
+		integer(c_int32_t) :: reserved(3)
 	end type tmpg_config
 
 	interface
