@@ -63,7 +63,9 @@ typedef struct tmpg_config {
     int32_t highorder;    /* nonzero: 4th-order field solver (<algorithm>
                              highorder; advance_b_halfstep_42 and
                              advance_e_fullstep_42, code/fields.F90:1039-1361) */
-    int32_t reserved[3];
+    int32_t pusher;       /* 0: Boris; 1: Vay (the reference's -Dvay build,
+                             code/particles_movedeposit.F90:1199-1223) */
+    int32_t reserved[2];
 } tmpg_config;
 
 typedef struct tmpg_ctx tmpg_ctx;

@@ -132,8 +132,8 @@ def read_state(path, cfg, dtype):
     return g, out
 
 
-def run_ref(dim, order, prec, inp, state, mode, cfg, ni, ne, maxhlf):
-    exe = os.path.join(REF_DIR, f"tristan_ref_{dim}d_dd{order}_{prec}")
+def run_ref(dim, order, prec, inp, state, mode, cfg, ni, ne, maxhlf, suffix=""):
+    exe = os.path.join(REF_DIR, f"tristan_ref_{dim}d_dd{order}_{prec}{suffix}")
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "input")
         open(fin, "w").write(inp)
@@ -226,8 +226,53 @@ def main_highorder():
             print(out, os.path.getsize(out))
 
 
+def main_vay():
+    """ref_{dim}d_ord{order}_vay.npz: one main-loop step and the mover alone
+    (driver mode 1) of the reference built with -Dvay (the Vay pusher);
+    2D order 1 and 3D order 3"""
+    for dim, order in ((2, 1), (3, 3)):
+        case = CASES[dim]
+        cfg = orc.make_cfg(dim, order, case["nx"], case["ny"], case["nz"], ntimes=PHYS["ntimes"])
+        rng = np.random.default_rng(7000 + 1000 * dim + order)
+        nz = case["nz"] if dim == 3 else 1
+        # fields 10x those of the other fixtures, so that the Vay and Boris
+        # pushes of the same state differ far beyond the parity bars
+        f_in = np.empty((6, nz, case["ny"], case["nx"]), dtype=np.float32)
+        f_in[:3] = rng.uniform(-0.5, 0.5, f_in[:3].shape)
+        f_in[3:] = rng.uniform(-1.0, 1.0, f_in[3:].shape)
+        pi = make_particles(rng, cfg, case["npart"])
+        pe = make_particles(rng, cfg, case["npart"] + 17)
+        maxptl = 20000
+        maxhlf = maxptl // 2
+        inp = input_text(dim, case["nx"], case["ny"], case["nz"], maxptl)
+        fix = dict(f_in=f_in, pi_in=pi, pe_in=pe)
+        sl = orc.interior_slices(cfg)
+        with tempfile.TemporaryDirectory() as td:
+            for prec, dt in (("f32", np.float32), ("f64", np.float64)):
+                f6 = orc.fill_from_interior(cfg, f_in.astype(dt), dt)
+                st = os.path.join(td, f"state_{prec}.bin")
+                write_state(st, f6, pi, pe, dt)
+                g, qi_, qe_ = run_ref(dim, order, prec, inp, st, 0, cfg, pi.shape[1], pe.shape[1], maxhlf, "_vay")
+                fix[f"step_{prec}"] = g[(slice(None),) + sl]
+                fix[f"step_pi_{prec}"] = qi_
+                fix[f"step_pe_{prec}"] = qe_
+                _, qi_, qe_ = run_ref(dim, order, prec, inp, st, 1, cfg, pi.shape[1], pe.shape[1], maxhlf, "_vay")
+                fix[f"move_pi_{prec}"] = qi_
+                fix[f"move_pe_{prec}"] = qe_
+                _, bi_, _ = run_ref(dim, order, prec, inp, st, 1, cfg, pi.shape[1], pe.shape[1], maxhlf)
+                d = np.linalg.norm(bi_[3:6] - qi_[3:6]) / np.linalg.norm(qi_[3:6])
+                print(f"{dim}D ord{order} {prec}: Boris vs Vay momenta after one push {d:.2e}")
+        params = dict(dim=dim, order=order, nx=case["nx"], ny=case["ny"], nz=nz, pusher="vay", **PHYS)
+        fix["params"] = np.array(repr(params))
+        out = os.path.join(HERE, f"ref_{dim}d_ord{order}_vay.npz")
+        np.savez_compressed(out, **fix)
+        print(out, os.path.getsize(out))
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["highorder"]:
         main_highorder()
+    elif sys.argv[1:] == ["vay"]:
+        main_vay()
     else:
         main()

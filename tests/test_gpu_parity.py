@@ -118,6 +118,67 @@ def test_highorder_step_vs_reference_fp64(tmpg, dim, order):
     ctx.close()
 
 
+VAY_CASES = [(2, 1), (3, 3)]
+
+
+@pytest.mark.parametrize("dim,order", VAY_CASES)
+def test_vay_mover_vs_reference_fp64(tmpg, dim, order):
+    """the Vay pusher (pusher = 1) against the reference built with -Dvay:
+    gather, momentum update and position advance (move_particles alone)"""
+    fx, p = load_fixture(dim, order, "_vay")
+    ctx = make_context(tmpg, dim, order, p["nx"], p["ny"], p["nz"], p, ntimes=p["ntimes"], pusher=1)
+    cfg = orc.make_cfg(dim, order, p["nx"], p["ny"], p["nz"])
+    ctx.set_fields(orc.fill_from_interior(cfg, fx["f_in"], np.float32)[:6])
+    ctx.set_particles(0, fx["pi_in"])
+    ctx.set_particles(1, fx["pe_in"])
+    ctx.move()
+    for s, key in ((0, "move_pi_f64"), (1, "move_pe_f64")):
+        got, ids = ctx.get_particles(s, with_ids=True)
+        _check_particles(ctx, got, ids, fx[key], f"vay species {s}")
+    ctx.deposit()
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,order", VAY_CASES)
+def test_vay_step_vs_reference_fp64(tmpg, dim, order):
+    fx, p = load_fixture(dim, order, "_vay")
+    ctx = make_context(tmpg, dim, order, p["nx"], p["ny"], p["nz"], p, ntimes=p["ntimes"], pusher=1)
+    cfg = orc.make_cfg(dim, order, p["nx"], p["ny"], p["nz"])
+    ctx.set_fields(orc.fill_from_interior(cfg, fx["f_in"], np.float32)[:6])
+    ctx.set_particles(0, fx["pi_in"])
+    ctx.set_particles(1, fx["pe_in"])
+    ctx.step(1)
+    f = _interior(ctx, ctx.get_fields())
+    j = _interior(ctx, ctx.get_currents())
+    ref = fx["step_f64"]
+    e_e, e_b, e_j = rel(f[:3], ref[:3]), rel(f[3:6], ref[3:6]), rel(j, ref[6:9])
+    print(f"vay dim{dim} ord{order}: E {e_e:.2e} B {e_b:.2e} J {e_j:.2e}")
+    assert e_e < TOL_FIELD and e_b < TOL_FIELD and e_j < TOL_FIELD
+    for s, key in ((0, "step_pi_f64"), (1, "step_pe_f64")):
+        got, ids = ctx.get_particles(s, with_ids=True)
+        _check_particles(ctx, got, ids, fx[key], f"vay species {s}")
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,order", VAY_CASES)
+def test_boris_is_not_vay_on_the_fixture(tmpg, dim, order):
+    """the same state pushed by Boris lands well outside the Vay bar, so the
+    parity above tells the two pushers apart"""
+    fx, p = load_fixture(dim, order, "_vay")
+    ctx = make_context(tmpg, dim, order, p["nx"], p["ny"], p["nz"], p, ntimes=p["ntimes"], pusher=0)
+    cfg = orc.make_cfg(dim, order, p["nx"], p["ny"], p["nz"])
+    ctx.set_fields(orc.fill_from_interior(cfg, fx["f_in"], np.float32)[:6])
+    ctx.set_particles(0, fx["pi_in"])
+    ctx.set_particles(1, fx["pe_in"])
+    ctx.move()
+    got, ids = ctx.get_particles(0, with_ids=True)
+    err = rel(sorted_by_id(got, ids)[3:6], fx["move_pi_f64"][3:6])
+    print(f"boris vs vay reference dim{dim} ord{order}: momenta {err:.2e}")
+    assert err > 20 * TOL_MOM
+    ctx.deposit()
+    ctx.close()
+
+
 @pytest.mark.parametrize("dim,order", CASES)
 def test_push_deposit_vs_oracle_fp64(tmpg, dim, order):
     """fused push+deposit (no filter) on a larger random state"""

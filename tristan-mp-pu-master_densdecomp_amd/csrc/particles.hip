@@ -47,7 +47,7 @@ struct TiledArgs {
     int FX, FY, FZ;         // staged field region
     int JX, JY, JZ;         // LDS current region (origin: tile origin - 2)
     float c, cinv;
-    int quirk2;
+    int flags;              // PUSH_GATHER2 | PUSH_VAY
     int remap;              // XCD-aware tile order
     const float* f6[6];
     float* j3[3];
@@ -109,15 +109,24 @@ __device__ __forceinline__ int64_t emit(const SpecArgs& s, const PackInfo& pk, i
     return -1;
 }
 
-// Boris push and position advance of p; returns the displacement
-template <int DIM, int ORD, class Src>
-__device__ __forceinline__ void push_one(const Src& src, int quirk2, float c, float cinv, float qm, int si, int sj,
+// push flags: the 2nd-order gather quirk; the Vay pusher instead of Boris
+constexpr int PUSH_GATHER2 = 1, PUSH_VAY = 2;
+
+// field gather and momentum push of p; returns the displacement.
+// VAY_OK = false compiles the Boris push only (the 3D cell-sum kernel, whose
+// register budget is tight; the host sends Vay runs to the other kernels)
+template <int DIM, int ORD, bool VAY_OK = true, class Src>
+__device__ __forceinline__ void push_one(const Src& src, int flags, float c, float cinv, float qm, int si, int sj,
                                          int sk, Part& p, float& dx, float& dy, float& dz)
 {
     float E[6];
-    gather<DIM, ORD>(src, quirk2 != 0, si, sj, sk, p.ox, p.oy, p.oz, E);
+    gather<DIM, ORD>(src, (flags & PUSH_GATHER2) != 0, si, sj, sk, p.ox, p.oy, p.oz, E);
     const float se = 0.5f * qm, sb = 0.5f * qm * cinv;
-    boris(c, cinv, se * E[0], se * E[1], se * E[2], sb * E[3], sb * E[4], sb * E[5], p.u, p.v, p.w, dx, dy, dz);
+    if (VAY_OK && (flags & PUSH_VAY))
+        vay(c, cinv, se * E[0], se * E[1], se * E[2], sb * E[3], sb * E[4], sb * E[5], p.u, p.v, p.w, dx, dy, dz);
+    else
+        boris(c, cinv, se * E[0], se * E[1], se * E[2], sb * E[3], sb * E[4], sb * E[5], p.u, p.v, p.w, dx, dy,
+              dz);
 }
 
 __device__ __forceinline__ void advance(Part& p, int dim, int nx, int ny, int nz, float dx, float dy, float dz)
@@ -193,7 +202,7 @@ __global__ void __launch_bounds__(512) k_push_tiled(TiledArgs a)
                 const int oi = p.i, oj = p.j, ok = p.k;
                 const float ox = p.ox, oy = p.oy, oz = p.oz;
                 float dx, dy, dz;
-                push_one<DIM, ORD>(src, a.quirk2, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo,
+                push_one<DIM, ORD>(src, a.flags, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo,
                                    (DIM == 3) ? lk + a.fo : 0, p, dx, dy, dz);
                 advance(p, DIM, a.nx, a.ny, a.nz, dx, dy, dz);
                 deposit<DIM, ORD>(snk, li + 2, lj + 2, (DIM == 3) ? lk + 2 : 0, ox, oy, oz, dx, dy, dz,
@@ -630,7 +639,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         if (active) {
             li = p.i - i0; lj = p.j - j0; lk = p.k - k0;
             ox = p.ox; oy = p.oy; oz = p.oz;
-            push_one<3, ORD>(src, a.quirk2, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx, dy,
+            push_one<3, ORD, false>(src, a.flags, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx, dy,
                              dz);
             advance(p, 3, a.nx, a.ny, a.nz, dx, dy, dz);
             q = sp.q * p.ch;
@@ -760,7 +769,7 @@ struct FlatArgs {
     PackInfo pk;
     Tiling tl;
     float c, cinv;
-    int quirk2;
+    int flags;              // PUSH_GATHER2 | PUSH_VAY
     GlobalSrc src;
     GlobalSink snk;
     SpecArgs sp;
@@ -805,7 +814,7 @@ __global__ void __launch_bounds__(256) k_push_flat(FlatArgs a)
     const int oi = p.i, oj = p.j, ok = p.k;
     const float ox = p.ox, oy = p.oy, oz = p.oz;
     float dx, dy, dz;
-    push_one<DIM, ORD>(a.src, a.quirk2, a.c, a.cinv, a.sp.qm, p.i, p.j, p.k, p, dx, dy, dz);
+    push_one<DIM, ORD>(a.src, a.flags, a.c, a.cinv, a.sp.qm, p.i, p.j, p.k, p, dx, dy, dz);
     advance(p, DIM, a.nx, a.ny, a.nz, dx, dy, dz);
     if (MODE == 0) deposit<DIM, ORD>(a.snk, oi, oj, ok, ox, oy, oz, dx, dy, dz, a.sp.q * p.ch);
     const int64_t td = tile_of(a.tl, p.i, p.j, p.k);
@@ -1008,6 +1017,11 @@ void dispatch(int dim, int order, bool ch, bool id, F&& f)
     }
 }
 
+int push_flags(const tmpg_ctx* ctx)
+{
+    return ((ctx->cfg.compat & TMPG_Q_GATHER2) ? PUSH_GATHER2 : 0) | (ctx->cfg.pusher == 1 ? PUSH_VAY : 0);
+}
+
 SpecArgs spec_args(tmpg_ctx* ctx, int s)
 {
     Species& sp = ctx->sp[s];
@@ -1034,7 +1048,7 @@ FlatArgs flat_args(tmpg_ctx* ctx, int s)
     a.tl = ctx->tl;
     a.c = ctx->cfg.c;
     a.cinv = 1.0f / ctx->cfg.c;
-    a.quirk2 = (ctx->cfg.compat & TMPG_Q_GATHER2) ? 1 : 0;
+    a.flags = push_flags(ctx);
     for (int q = 0; q < 6; ++q) a.src.f[q] = (g.dim == 3) ? ctx->prim[q] : ctx->f[EX + q];
     a.src.nx = g.nx; a.src.ny = g.ny; a.src.nz = g.nz;
     for (int q = 0; q < 3; ++q) a.snk.j[q] = ctx->f[JX + q];
@@ -1318,7 +1332,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
         a.JZ = (g.dim == 3) ? tl.tz + 5 : 1;
         a.c = ctx->cfg.c;
         a.cinv = 1.0f / ctx->cfg.c;
-        a.quirk2 = (ctx->cfg.compat & TMPG_Q_GATHER2) ? 1 : 0;
+        a.flags = push_flags(ctx);
         a.remap = (tl.ntiles % 8 == 0) ? 1 : 0;
         for (int q = 0; q < 6; ++q) a.f6[q] = (g.dim == 3) ? ctx->prim[q] : ctx->f[EX + q];
         for (int q = 0; q < 3; ++q) a.j3[q] = ctx->f[JX + q];
@@ -1337,7 +1351,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
         const int ncell_tile = tl.tx * tl.ty * tl.tz;
         const CellsLds CL = cells_lds(a.FX, a.FY, a.FZ, a.JX, a.JY, a.JZ, ncell_tile);
         const bool cells = g.dim == 3 && ctx->use_cell_sums && ncell_tile <= CELLS_T / 2 * 2 &&
-                           CL.bytes <= 160 * 1024;
+                           CL.bytes <= 160 * 1024 && ctx->cfg.pusher == 0;
         // order 3 only: there a direct deposit (about 300 LDS atomics per
         // particle) costs more than draining the records in passes; at
         // order 2 the direct deposits of the excess particles are cheaper

@@ -264,6 +264,39 @@ __device__ __forceinline__ void boris(float c, float cinv, float ex0, float ey0,
     dz = w * g * c;
 }
 
+// ------------------------------------------------------------------ Vay
+// code/particles_movedeposit.F90:1199-1223 (under -Dvay; the same block in
+// every mover), then the common update of :1251-1267.  Same inputs and
+// outputs as boris().
+__device__ __forceinline__ void vay(float c, float cinv, float ex0, float ey0, float ez0,
+                                    float bx0, float by0, float bz0,
+                                    float& u, float& v, float& w,
+                                    float& dx, float& dy, float& dz)
+{
+    float g = 1.0f / sqrtf(1.0f + u * u + v * v + w * w);
+    const float vx0 = c * u * g, vy0 = c * v * g, vz0 = c * w * g;
+    const float u1 = c * u + 2.0f * ex0 + vy0 * bz0 - vz0 * by0;
+    const float v1 = c * v + 2.0f * ey0 + vz0 * bx0 - vx0 * bz0;
+    const float w1 = c * w + 2.0f * ez0 + vx0 * by0 - vy0 * bx0;
+    const float ustar = cinv * (u1 * bx0 + v1 * by0 + w1 * bz0);
+    const float b2 = bx0 * bx0 + by0 * by0 + bz0 * bz0;
+    const float sig = cinv * cinv * (c * c + u1 * u1 + v1 * v1 + w1 * w1) - b2;
+    g = 1.0f / sqrtf(0.5f * (sig + sqrtf(sig * sig + 4.0f * (b2 + ustar * ustar))));
+    const float tx = bx0 * g, ty = by0 * g, tz = bz0 * g;
+    const float f = 1.0f / (1.0f + tx * tx + ty * ty + tz * tz);
+    const float ut = u1 * tx + v1 * ty + w1 * tz;
+    const float u0 = f * (u1 + ut * tx + v1 * tz - w1 * ty);
+    const float v0 = f * (v1 + ut * ty + w1 * tx - u1 * tz);
+    const float w0 = f * (w1 + ut * tz + u1 * ty - v1 * tx);
+    u = u0 * cinv;
+    v = v0 * cinv;
+    w = w0 * cinv;
+    g = c * __frsqrt_rn(c * c + u0 * u0 + v0 * v0 + w0 * w0);
+    dx = u * g * c;
+    dy = v * g * c;
+    dz = w * g * c;
+}
+
 // advance offset o by dl; returns the cell shift (-1, 0, +1) and the new
 // offset in [0,1)
 __device__ __forceinline__ int advance_offset(float& o, float dl)

@@ -137,6 +137,7 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
     *out = nullptr;
     if (cfg->dim != 2 && cfg->dim != 3) { g_create_err = "dim must be 2 or 3"; return TMPG_EINVAL; }
     if (cfg->order < 1 || cfg->order > 3) { g_create_err = "order must be 1, 2 or 3"; return TMPG_EINVAL; }
+    if (cfg->pusher < 0 || cfg->pusher > 1) { g_create_err = "pusher must be 0 (Boris) or 1 (Vay)"; return TMPG_EINVAL; }
     const int nz = cfg->dim == 3 ? cfg->nz : 1;
     if (cfg->nx < 4 || cfg->ny < 4 || nz < 1 || (cfg->dim == 3 && nz < 4)) {
         g_create_err = "grid too small (need at least 4 cells per periodic axis)";

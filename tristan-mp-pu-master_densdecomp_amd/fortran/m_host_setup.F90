@@ -22,7 +22,7 @@ module m_host_setup
 		integer :: dim = 2, order = 3
 		integer :: nx = 100, ny = 100, nz = 1
 		integer :: last = 100, ntimes = 4, highorder = 1, interval = 10
-		integer :: fused = 1, seed = 0
+		integer :: fused = 1, seed = 0, pusher = 0
 		real(8) :: c = 0.45d0, corr = 1.025d0
 		real(8) :: ppc0 = 1d0, c_omp = 0d0, gamma0 = 1d0, delgam = 1d-8, me = 1d0, mi = 20d0
 		real(8) :: temperature_ratio = 1d0
@@ -61,6 +61,7 @@ contains
 		call gi("gpu", "order", 3, hp%order)
 		call gi("gpu", "fused", 1, hp%fused)
 		call gi("gpu", "seed", 0, hp%seed)
+		call gi("gpu", "pusher", 0, hp%pusher)
 		call tmpg_check(tmpg_input_close(inp), "tmpg_input_close", c_null_ptr)
 		if (hp%dim == 2) hp%nz = 1
 		call normalise(hp)
@@ -118,6 +119,7 @@ contains
 		cfg%tile = 0
 		cfg%sort_every = 0
 		cfg%highorder = merge(1, 0, hp%highorder /= 0)
+		cfg%pusher = hp%pusher
 		cfg%reserved = 0
 	end subroutine make_config
 

@@ -21,7 +21,8 @@ module m_tmpg
 		integer(c_int32_t) :: tile(3)
 		integer(c_int32_t) :: sort_every
 		integer(c_int32_t) :: highorder
-		integer(c_int32_t) :: reserved(3)
+		integer(c_int32_t) :: pusher
+		integer(c_int32_t) :: reserved(2)
 	end type tmpg_config
 
 	interface

@@ -60,7 +60,8 @@ class Config(ctypes.Structure):
         ("tile", ctypes.c_int32 * 3),
         ("sort_every", ctypes.c_int32),
         ("highorder", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("pusher", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -101,7 +102,7 @@ class Context:
 
     def __init__(self, dim, order, nx, ny, nz=1, ntimes=4, compat=COMPAT_DEFAULT, c=0.45, corr=1.025,
                  qm=(1.0, -1.0), q=(1.0, -1.0), mass=(1.0, 1.0), capacity=(0, 0), track_ids=False,
-                 tile=(0, 0, 0), sort_every=0, highorder=0):
+                 tile=(0, 0, 0), sort_every=0, highorder=0, pusher=0):
         self.lib = load_library()
         cfg = Config()
         cfg.dim, cfg.order, cfg.nx, cfg.ny, cfg.nz = dim, order, nx, ny, (nz if dim == 3 else 1)
@@ -113,6 +114,7 @@ class Context:
             cfg.tile[a] = tile[a]
         cfg.sort_every = sort_every
         cfg.highorder = int(bool(highorder))
+        cfg.pusher = int(pusher)
         self.cfg = cfg
         self.dim, self.order = dim, order
         self.nx, self.ny, self.nz = nx, ny, (nz if dim == 3 else 1)
