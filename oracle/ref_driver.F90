@@ -15,7 +15,9 @@
 !   state-out  : same layout + curx,cury,curz appended after the fields
 !   mode       : 0 full step (main-loop order), 1 mover only,
 !                2 deposit only (reset, deposit, fold), 3 filter only
-!                (state-in currents are read after the fields),
+!                (state-in currents are read after the fields;
+!                apply_filter2_opt in a -Dfilter2 build, as the main loop
+!                at code/tristanmainloop.F90:217-219),
 !                4 Yee only (B half, B half, E), 5 deposit + fold + filter
 !   optional   : <history-file> <every>: in mode 0, every <every> steps one
 !                line "step E_energy B_energy KE_ions KE_electrons" in FP64:
@@ -145,7 +147,11 @@ program tristan_ref
 			call deposit_particles()
 			call exchange_particles()
 			call exchange_current()
+#ifdef filter2
+			call apply_filter2_opt()
+#else
 			call apply_filter1_opt()
+#endif
 			call add_current()
 			call inject_others()
 			call exchange_particles()
@@ -163,7 +169,11 @@ program tristan_ref
 			call exchange_particles()
 			call inject_others()
 		case (3)
+#ifdef filter2
+			call apply_filter2_opt()
+#else
 			call apply_filter1_opt()
+#endif
 		case (4)
 			call bc_b1()
 			call bc_e1()
@@ -178,7 +188,11 @@ program tristan_ref
 			call deposit_particles()
 			call exchange_particles()
 			call exchange_current()
+#ifdef filter2
+			call apply_filter2_opt()
+#else
 			call apply_filter1_opt()
+#endif
 			call inject_others()
 			call exchange_particles()
 			call inject_others()

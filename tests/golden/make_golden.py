@@ -269,8 +269,62 @@ def main_vay():
         print(out, os.path.getsize(out))
 
 
+def main_filter2():
+    """ref_{dim}d_ord{order}_filter2.npz: the reference built with -Dfilter2
+    (apply_filter2_opt, code/optimized_filters.F90): the filter alone on
+    random currents (driver mode 3) and one full main-loop step; 3D order 3
+    and 2.  (In 2D the reference never creates the MPI datatypes filter2
+    copies with: code/fields.F90:362-368 calls create_MPI_filter_datatypes
+    only #ifndef twoD, so a 2D filter2 build stops in its first MPI call.)"""
+    for dim, order in ((3, 3), (3, 2)):
+        case = CASES[dim]
+        cfg = orc.make_cfg(dim, order, case["nx"], case["ny"], case["nz"], ntimes=PHYS["ntimes"])
+        rng = np.random.default_rng(9000 + 1000 * dim + order)
+        nz = case["nz"] if dim == 3 else 1
+        f_in = np.empty((6, nz, case["ny"], case["nx"]), dtype=np.float32)
+        f_in[:3] = rng.uniform(-0.05, 0.05, f_in[:3].shape)
+        f_in[3:] = rng.uniform(-0.1, 0.1, f_in[3:].shape)
+        j_in = rng.uniform(-1.0, 1.0, (3, nz, case["ny"], case["nx"])).astype(np.float32)
+        pi = make_particles(rng, cfg, case["npart"])
+        pe = make_particles(rng, cfg, case["npart"] + 17)
+        maxptl = 20000
+        maxhlf = maxptl // 2
+        inp = input_text(dim, case["nx"], case["ny"], case["nz"], maxptl)
+        fix = dict(f_in=f_in, j_in=j_in, pi_in=pi, pe_in=pe)
+        sl = orc.interior_slices(cfg)
+        with tempfile.TemporaryDirectory() as td:
+            for prec, dt in (("f32", np.float32), ("f64", np.float64)):
+                f6 = orc.fill_from_interior(cfg, f_in.astype(dt), dt)
+                st = os.path.join(td, f"state_{prec}.bin")
+                write_state(st, f6, pi, pe, dt)
+                g, qi_, qe_ = run_ref(dim, order, prec, inp, st, 0, cfg, pi.shape[1], pe.shape[1], maxhlf,
+                                      "_filter2")
+                fix[f"step_{prec}"] = g[(slice(None),) + sl]
+                fix[f"step_pi_{prec}"] = qi_
+                fix[f"step_pe_{prec}"] = qe_
+                # filter alone: currents after the fields in the state file
+                j3 = orc.fill_from_interior(cfg, j_in.astype(dt), dt)[:3]
+                stj = os.path.join(td, f"statej_{prec}.bin")
+                with open(stj, "wb") as fh:
+                    fh.write(np.ascontiguousarray(f6, dtype=dt).tobytes())
+                    fh.write(np.ascontiguousarray(j3, dtype=dt).tobytes())
+                    fh.write(np.array([pi.shape[1], pe.shape[1]], dtype=np.int32).tobytes())
+                    for pp in (pi, pe):
+                        for r in range(7):
+                            fh.write(np.ascontiguousarray(pp[r], dtype=dt).tobytes())
+                g, _, _ = run_ref(dim, order, prec, inp, stj, 3, cfg, pi.shape[1], pe.shape[1], maxhlf, "_filter2")
+                fix[f"filt_{prec}"] = g[(slice(6, 9),) + sl]
+        params = dict(dim=dim, order=order, nx=case["nx"], ny=case["ny"], nz=nz, filter="filter2", **PHYS)
+        fix["params"] = np.array(repr(params))
+        out = os.path.join(HERE, f"ref_{dim}d_ord{order}_filter2.npz")
+        np.savez_compressed(out, **fix)
+        print(out, os.path.getsize(out))
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["highorder"]:
+    if sys.argv[1:] == ["filter2"]:
+        main_filter2()
+    elif sys.argv[1:] == ["highorder"]:
         main_highorder()
     elif sys.argv[1:] == ["vay"]:
         main_vay()

@@ -118,6 +118,47 @@ def test_highorder_step_vs_reference_fp64(tmpg, dim, order):
     ctx.close()
 
 
+F2_CASES = [(3, 2), (3, 3)]
+
+
+@pytest.mark.parametrize("dim,order", F2_CASES)
+def test_filter_vs_filter2_reference_fp64(tmpg, dim, order):
+    """the library's filter against the reference's separable filter2
+    (apply_filter2_opt, code/optimized_filters.F90; -Dfilter2 build) on
+    random currents: the same operator, summed in another order"""
+    fx, p = load_fixture(dim, order, "_filter2")
+    ctx = make_context(tmpg, dim, order, p["nx"], p["ny"], p["nz"], p, ntimes=p["ntimes"])
+    cfg = orc.make_cfg(dim, order, p["nx"], p["ny"], p["nz"])
+    ctx.set_currents(orc.fill_from_interior(cfg, fx["j_in"], np.float32)[:3])
+    ctx.filter(p["ntimes"])
+    j = _interior(ctx, ctx.get_currents())
+    err = rel(j, fx["filt_f64"])
+    print(f"filter2 dim{dim} ord{order}: J {err:.2e}")
+    assert err < TOL_FIELD
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,order", F2_CASES)
+def test_step_vs_filter2_reference_fp64(tmpg, dim, order):
+    fx, p = load_fixture(dim, order, "_filter2")
+    ctx = make_context(tmpg, dim, order, p["nx"], p["ny"], p["nz"], p, ntimes=p["ntimes"])
+    cfg = orc.make_cfg(dim, order, p["nx"], p["ny"], p["nz"])
+    ctx.set_fields(orc.fill_from_interior(cfg, fx["f_in"], np.float32)[:6])
+    ctx.set_particles(0, fx["pi_in"])
+    ctx.set_particles(1, fx["pe_in"])
+    ctx.step(1)
+    f = _interior(ctx, ctx.get_fields())
+    j = _interior(ctx, ctx.get_currents())
+    ref = fx["step_f64"]
+    e_e, e_b, e_j = rel(f[:3], ref[:3]), rel(f[3:6], ref[3:6]), rel(j, ref[6:9])
+    print(f"filter2 step dim{dim} ord{order}: E {e_e:.2e} B {e_b:.2e} J {e_j:.2e}")
+    assert e_e < TOL_FIELD and e_b < TOL_FIELD and e_j < TOL_FIELD
+    for s, key in ((0, "step_pi_f64"), (1, "step_pe_f64")):
+        got, ids = ctx.get_particles(s, with_ids=True)
+        _check_particles(ctx, got, ids, fx[key], f"filter2 species {s}")
+    ctx.close()
+
+
 VAY_CASES = [(2, 1), (3, 3)]
 
 
