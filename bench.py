@@ -55,6 +55,19 @@ WORKLOADS = {
                                        c=0.45, c_omp=10.0, me=1.0, mi=1.0, ntimes=4,
                                        load=[(0, 4, -0.5, 2.5e-3), (1, 4, -0.5, 2.5e-3),
                                              (0, 4, 0.5, 2.5e-3), (1, 4, 0.5, 2.5e-3)]),
+    # 2D pair Weibel, 2nd order, 1024^2: the plain run at 32 ppc per species
+    # and the 2-ppc run with the learned denoiser on 8 filter passes (its
+    # weights do not change the time; the physics comparison is in
+    # train_denoiser.parity_study)
+    "weibel2d_ord2_1024x1024_ppc32": dict(dim=2, order=2, nx=1024, ny=1024, nz=1, ppc=32, gb=0.5, theta=2.5e-3,
+                                          c=0.45, c_omp=5.0, me=1.0, mi=1.0, ntimes=4,
+                                          load=[(0, 16, -0.5, 2.5e-3), (1, 16, -0.5, 2.5e-3),
+                                                (0, 16, 0.5, 2.5e-3), (1, 16, 0.5, 2.5e-3)]),
+    "weibel2d_ord2_1024x1024_ppc2_denoised": dict(dim=2, order=2, nx=1024, ny=1024, nz=1, ppc=2, gb=0.5,
+                                                  theta=2.5e-3, c=0.45, c_omp=5.0, me=1.0, mi=1.0, ntimes=4,
+                                                  denoise=dict(base_passes=8),
+                                                  load=[(0, 1, -0.5, 2.5e-3), (1, 1, -0.5, 2.5e-3),
+                                                        (0, 1, 0.5, 2.5e-3), (1, 1, 0.5, 2.5e-3)]),
 }
 
 
@@ -81,6 +94,11 @@ def make_context(w, sort_every=0):
                        capacity=(cap, cap), sort_every=sort_every)
     for sp, ppc, gb, theta in w["load"]:
         ctx.load_uniform(sp, ppc, gb, theta, 7 + 2 * sp + (gb > 0))
+    if w.get("denoise"):
+        m = tmpg.Model(w["dim"], seed=0)
+        m.set_base_passes(w["denoise"]["base_passes"])
+        m.set_scale(1.0)
+        ctx.set_denoiser(m)
     return ctx, ncell
 
 
@@ -234,7 +252,8 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             print(f"cpu baseline failed: {e}", file=sys.stderr)
     out = {
-        "metric": f"particle-steps/s ({w['dim']}D ord-{w['order']} full step)",
+        "metric": f"particle-steps/s ({w['dim']}D ord-{w['order']} full step"
+                  f"{', learned denoiser' if w.get('denoise') else ''})",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
