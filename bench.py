@@ -63,6 +63,18 @@ WORKLOADS = {
                                           c=0.45, c_omp=5.0, me=1.0, mi=1.0, ntimes=4,
                                           load=[(0, 16, -0.5, 2.5e-3), (1, 16, -0.5, 2.5e-3),
                                                 (0, 16, 0.5, 2.5e-3), (1, 16, 0.5, 2.5e-3)]),
+    # 3D pair Weibel at scale, 3rd order, 256^3: the plain run at 32 ppc per
+    # species (1.07e9 particles) and the 2-ppc run with the learned denoiser
+    # on 4 filter passes
+    "weibel3d_ord3_256cube_ppc32": dict(dim=3, order=3, nx=256, ny=256, nz=256, ppc=32, gb=0.5, theta=2.5e-3,
+                                        c=0.45, c_omp=8.0, me=1.0, mi=1.0, ntimes=4,
+                                        load=[(0, 16, -0.5, 2.5e-3), (1, 16, -0.5, 2.5e-3),
+                                              (0, 16, 0.5, 2.5e-3), (1, 16, 0.5, 2.5e-3)]),
+    "weibel3d_ord3_256cube_ppc2_denoised": dict(dim=3, order=3, nx=256, ny=256, nz=256, ppc=2, gb=0.5,
+                                                theta=2.5e-3, c=0.45, c_omp=8.0, me=1.0, mi=1.0, ntimes=4,
+                                                denoise=dict(base_passes=4),
+                                                load=[(0, 1, -0.5, 2.5e-3), (1, 1, -0.5, 2.5e-3),
+                                                      (0, 1, 0.5, 2.5e-3), (1, 1, 0.5, 2.5e-3)]),
     "weibel2d_ord2_1024x1024_ppc2_denoised": dict(dim=2, order=2, nx=1024, ny=1024, nz=1, ppc=2, gb=0.5,
                                                   theta=2.5e-3, c=0.45, c_omp=5.0, me=1.0, mi=1.0, ntimes=4,
                                                   denoise=dict(base_passes=8),
