@@ -232,7 +232,7 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
         return
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (one periodic domain per GPU); "
+        print("bench.py measures one GPU: the library has no multi-GPU path (one periodic domain per GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     w = WORKLOADS[a.workload]

@@ -1,12 +1,12 @@
 """The trained denoiser in the loop: a 2D pair-Weibel run at 2 particles per
 cell per species with the denoiser against the same run at 32 particles per
-cell (the plain method, the judge) and the plain 2-ppc run.
+cell (the plain method, the yardstick) and the plain 2-ppc run.
 
 The model is trained on the GPU with the library's kernels, by the recipe of
 train_denoiser.RECIPE: pairs from three 96x96 runs at 32 ppc (target: the
 filtered current) and their 1-in-16 subsamples (input: the raw current),
 8 filter passes under the curl-form correction, amplitude weight 8.  It is
-judged on 192x192 cells over 8 seeds (11-18) that training never saw.
+compared on 192x192 cells over 8 seeds (11-18) that training never saw.
 
 Bars: the B-energy growth rate within 5% of the 32-ppc run's and closer to
 it than the plain 2-ppc run's (which is about 7% low); the peak B energy

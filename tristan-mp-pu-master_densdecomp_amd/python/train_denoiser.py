@@ -20,7 +20,7 @@ method's and the network improves the transverse part.
 
 The recipe that reproduces a 32-ppc pair Weibel run from 2 ppc per species
 (8 filter passes under the correction, amplitude weight 8, pairs from three
-96x96 runs), judged on 1024x1024 cells:
+96x96 runs), compared on 1024x1024 cells:
 
     python train_denoiser.py --nx 96 --ny 96 --ppc-hi 32 --ppc-lo 2 --steps 600 --every 30 \
         --seeds 0 1 2 --iters 3000 --study 1024
@@ -249,7 +249,7 @@ def growth_rate(laps, eb, lo_frac=1e-3, hi_frac=0.1):
 
 def compare_runs(model, dim=2, order=2, nx=96, ny=96, nz=1, ppc_hi=32, ppc_lo=4, gb=0.5, theta=2.5e-3,
                  steps=600, every=10, seed=11, c_omp=5.0, ntimes=4):
-    """Weibel runs of one setup: high-ppc plain (the judge), low-ppc plain and
+    """Weibel runs of one setup: high-ppc plain (the yardstick), low-ppc plain and
     low-ppc with the denoiser in place of the filter.  Returns per run the
     energy history, B-energy growth rate, saturated B energy and GPU time
     per lap (the library's own event timings)."""
@@ -349,7 +349,7 @@ def main():
     ap.add_argument("--base-passes", type=int, default=RECIPE["base_passes"])
     ap.add_argument("--amplitude-weight", type=float, default=RECIPE["amplitude_weight"])
     ap.add_argument("--study", type=int, default=0, metavar="N",
-                    help="then judge the model on N x N cells (8 seeds): growth rate, peak B energy and GPU time "
+                    help="then compare the model on N x N cells (8 seeds): growth rate, peak B energy and GPU time "
                          "of the denoised low-ppc run against the plain high- and low-ppc runs")
     ap.add_argument("--out", default="")
     a = ap.parse_args()
