@@ -24,7 +24,7 @@ mx0 = 64
 my0 = 48
 mz0 = 1
 <algorithm>
-highorder = 0
+highorder = {highorder}
 Corr = 1.025
 ntimes = 4
 <output>
@@ -42,13 +42,15 @@ temperature_ratio = 1
 order = 2
 fused = {fused}
 seed = 3
+pusher = {pusher}
 """
 
 
-def run(tmp_path, fused, last=30):
-    inp = tmp_path / f"input{fused}"
-    inp.write_text(INPUT.format(last=last, fused=fused))
-    out = tmp_path / f"energy{fused}.txt"
+def run(tmp_path, fused, last=30, highorder=0, pusher=0):
+    tag = f"{fused}{highorder}{pusher}"
+    inp = tmp_path / f"input{tag}"
+    inp.write_text(INPUT.format(last=last, fused=fused, highorder=highorder, pusher=pusher))
+    out = tmp_path / f"energy{tag}.txt"
     r = subprocess.run([EXE, "-i", str(inp), "-o", str(out)], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
@@ -67,3 +69,23 @@ def test_fortran_host_runs_and_sequences_agree(tmp_path):
     # fused and split laps: same physics up to FP32 summation order
     np.testing.assert_allclose(a[:, 3:], b[:, 3:], rtol=1e-5)
     np.testing.assert_allclose(a[:, 1:3], b[:, 1:3], rtol=2e-2, atol=1e-12 * a[0, 3])
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+def test_fortran_host_passes_solver_and_pusher_choices(tmp_path):
+    """highorder = 1 (4th-order field solver) and <gpu> pusher = 1 (Vay)
+    reach the library, and the fused and split laps still agree.  The 4th-
+    order solver visibly changes the field energies; in this weak-field start
+    Boris and Vay differ far below them (their difference is pinned by the
+    parity tests against the reference's -Dvay build)."""
+    base, out = run(tmp_path, 1, last=60)
+    assert "field solver 2nd order" in out and "pusher Boris" in out
+    for ho, pu in ((1, 0), (0, 1)):
+        a, out = run(tmp_path, 1, last=60, highorder=ho, pusher=pu)
+        b, _ = run(tmp_path, 0, last=60, highorder=ho, pusher=pu)
+        assert ("4th order" in out) == (ho == 1) and ("pusher Vay" in out) == (pu == 1)
+        np.testing.assert_allclose(a[:, 3:], b[:, 3:], rtol=1e-5)
+        assert np.all(np.isfinite(a))
+        if ho:
+            # the 4th-order solver changes the fields well beyond round-off
+            assert not np.allclose(a[:, 1:3], base[:, 1:3], rtol=1e-6, atol=0)

@@ -52,6 +52,8 @@ program tristan_gpu
 	call tmpg_check(tmpg_count_particles(ctx, 1_c_int, ne), "count", ctx)
 	write(*, '(a,i0,a,i0,a,i0,a,i0,a,i0,a,i0)') "grid ", hp%nx, " x ", hp%ny, " x ", hp%nz, &
 		"  order ", hp%order, "  ions ", ni, "  electrons ", ne
+	write(*, '(a,a,a,a)') "field solver ", trim(merge("4th order (highorder)", "2nd order            ", &
+		hp%highorder /= 0)), "  pusher ", trim(merge("Vay  ", "Boris", hp%pusher == 1))
 	u = 0
 	if (len_trim(outfile) > 0) then
 		open(newunit=u, file=trim(outfile), status="replace", action="write")
