@@ -219,6 +219,16 @@ __global__ void __launch_bounds__(256) k_primal(const float* __restrict__ ex, co
     pbz[o] = 0.5f * (0.5f * (bz[o] + bz[I(im, j, k)]) + 0.5f * (bz[I(i, jm, k)] + bz[I(im, jm, k)]));
 }
 
+// the three current components zeroed in one launch
+__global__ void k_zero3(float* __restrict__ a, float* __restrict__ b, float* __restrict__ c, int64_t n)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        a[t] = 0.0f;
+        b[t] = 0.0f;
+        c[t] = 0.0f;
+    }
+}
+
 __global__ void k_zero(float* __restrict__ a, int64_t n)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,8 +422,7 @@ int fields_zero_j(tmpg_ctx* ctx)
 {
     const Grid& g = ctx->g;
     int blocks = (int)std::min<int64_t>((g.n + 255) / 256, 8192);
-    for (int q = 0; q < 3; ++q)
-        hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[JX + q], g.n);
+    hipLaunchKernelGGL(k_zero3, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[JX], ctx->f[JY], ctx->f[JZ], g.n);
     TMPG_HIP_CHECK(ctx, hipGetLastError());
     return TMPG_OK;
 }

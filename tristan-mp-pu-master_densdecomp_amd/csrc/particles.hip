@@ -970,6 +970,32 @@ __global__ void __launch_bounds__(SORT_THREADS) k_tile_sort(PBuf in, const int32
     }
 }
 
+// before a push: the output tile counts of both species and their new
+// overflow-list counts start at zero (one launch instead of four memsets)
+__global__ void k_reset_counts(int32_t* __restrict__ cb0, int32_t* __restrict__ cb1, int64_t ntiles,
+                               int32_t* __restrict__ oc0, int32_t* __restrict__ oc1)
+{
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t t = t0; t < ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+        if (cb0) cb0[t] = 0;
+        if (cb1) cb1[t] = 0;
+    }
+    if (t0 == 0) {
+        if (oc0) oc0[1] = 0;
+        if (oc1) oc1[1] = 0;
+    }
+}
+
+// after a push: the overflow list just filled (count in slot 1) becomes the
+// one the next push serves (slot 0)
+__global__ void k_promote_lists(int32_t* __restrict__ oc0, int32_t* __restrict__ oc1)
+{
+    if (threadIdx.x == 0) {
+        if (oc0) oc0[0] = oc0[1];
+        if (oc1) oc1[0] = oc1[1];
+    }
+}
+
 // compaction: tile segments of `in` -> consecutive positions of `out`
 // starting at off[t]
 template <int DIM, bool HAS_CH, bool HAS_ID>
@@ -1310,12 +1336,12 @@ int particles_push(tmpg_ctx* ctx, int mode)
     for (int s = 0; s < 2; ++s) any |= ctx->sp[s].n > 0;
     if (!any) return TMPG_OK;
     if (mode != 2) {
-        for (int s = 0; s < 2; ++s) {
-            Species& sp = ctx->sp[s];
-            if (!sp.cnt_b) continue;
-            TMPG_HIP_CHECK(ctx, hipMemsetAsync(sp.cnt_b, 0, tl.ntiles * sizeof(int32_t), ctx->stream));
-            TMPG_HIP_CHECK(ctx, hipMemsetAsync(sp.ocnt + 1, 0, sizeof(int32_t), ctx->stream));
-        }
+        Species& s0 = ctx->sp[0];
+        Species& s1 = ctx->sp[1];
+        const int blocks = (int)std::min<int64_t>((tl.ntiles + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_reset_counts, dim3(blocks), dim3(256), 0, ctx->stream, s0.cnt_b, s1.cnt_b, tl.ntiles,
+                           s0.cnt_b ? s0.ocnt : nullptr, s1.cnt_b ? s1.ocnt : nullptr);
+        TMPG_HIP_CHECK(ctx, hipGetLastError());
     }
     if (mode == 0) {
         // the two species share the tile's staged fields: one launch
@@ -1414,13 +1440,17 @@ int particles_push(tmpg_ctx* ctx, int mode)
         }
     }
     if (mode != 2) {
+        int32_t* oc[2] = {nullptr, nullptr};
         for (int s = 0; s < 2; ++s) {
             Species& sp = ctx->sp[s];
             if (!sp.cnt_a) continue;
             swap_buffers(sp);
-            // the new overflow list's count moves to slot 0
-            TMPG_HIP_CHECK(ctx, hipMemcpyAsync(sp.ocnt, sp.ocnt + 1, sizeof(int32_t), hipMemcpyDeviceToDevice,
-                                               ctx->stream));
+            oc[s] = sp.ocnt;
+        }
+        // the new overflow lists' counts move to slot 0
+        if (oc[0] || oc[1]) {
+            hipLaunchKernelGGL(k_promote_lists, dim3(1), dim3(64), 0, ctx->stream, oc[0], oc[1]);
+            TMPG_HIP_CHECK(ctx, hipGetLastError());
         }
     }
     return TMPG_OK;
