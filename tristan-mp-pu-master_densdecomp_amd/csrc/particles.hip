@@ -639,8 +639,8 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         if (active) {
             li = p.i - i0; lj = p.j - j0; lk = p.k - k0;
             ox = p.ox; oy = p.oy; oz = p.oz;
-            push_one<3, ORD, false>(src, a.flags, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx, dy,
-                             dz);
+            push_one<3, ORD, false>(src, a.flags, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx,
+                                    dy, dz);
             advance(p, 3, a.nx, a.ny, a.nz, dx, dy, dz);
             q = sp.q * p.ch;
             fits = fits4<ORD>(oy, dy) && fits4<ORD>(oz, dz);
