@@ -624,7 +624,6 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     // ---- phase A: push, emit, record or deposit at once.  A tile with more
     // particles than records drains the records whenever a round of NT
     // might not fit (the test is block-uniform: read between two barriers)
-    const bool multipass = min(a.sp[0].cnt_a[t], a.sp[0].tcap) + min(a.sp[1].cnt_a[t], a.sp[1].tcap) > CELLS_R;
     // one round of phase A: NT particles of species `sp` starting at p0;
     // `pn`/`an` carry the prefetched particle of the next round
     auto round = [&](const SpecArgs& sp, int n, int64_t base, int64_t slots, int p0, Part& pn, bool& an) {
@@ -716,6 +715,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     // are drained, and phase A resumes at the block-uniform point
     // (species s, round p0); the test reads the record count between two
     // barriers
+    const bool multipass = min(a.sp[0].cnt_a[t], a.sp[0].tcap) + min(a.sp[1].cnt_a[t], a.sp[1].tcap) > CELLS_R;
     int s = 0, p0 = 0;
 #pragma unroll 1
     for (;;) {
