@@ -28,6 +28,10 @@ CASES = {
     # 2D: 256^2 cells, 8 particles per cell per species, 1st order, c/wp = 10
     # (the size of the reference's CPU-runnable 2D configuration)
     "weibel2d_ord1_256": dict(dim=2, order=1, nx=256, ny=256, nz=1, ppc=8, steps=1000, seed=73, c_omp=10.0),
+    # 3D: 16^3 cells, 12 particles per cell per species, 3rd order: 6144
+    # particles per 8x8x4 tile, beyond the push's record space (the
+    # multi-pass push)
+    "weibel3d_ord3_dense": dict(dim=3, order=3, nx=16, ny=16, nz=16, ppc=12, steps=300, seed=74),
 }
 PHYS = dict(c=0.45, corr=1.025, ntimes=4, c_omp=5.0, gamma0=0.5, me=1.0, mi=1.0, gb=0.5, theta=2.5e-3)
 EVERY = 10
