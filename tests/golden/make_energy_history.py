@@ -32,6 +32,10 @@ CASES = {
     # particles per 8x8x4 tile, beyond the push's record space (the
     # multi-pass push)
     "weibel3d_ord3_dense": dict(dim=3, order=3, nx=16, ny=16, nz=16, ppc=12, steps=300, seed=74),
+    # 2D order 2 with the 4th-order field solver (highorder = 1)
+    "weibel2d_ord2_highorder": dict(dim=2, order=2, nx=64, ny=48, nz=1, ppc=8, steps=600, seed=75, highorder=1),
+    # 2D order 1 with the Vay pusher (the reference built with -Dvay)
+    "weibel2d_ord1_vay": dict(dim=2, order=1, nx=64, ny=48, nz=1, ppc=8, steps=600, seed=76, pusher="vay"),
 }
 PHYS = dict(c=0.45, corr=1.025, ntimes=4, c_omp=5.0, gamma0=0.5, me=1.0, mi=1.0, gb=0.5, theta=2.5e-3)
 EVERY = 10
@@ -50,7 +54,7 @@ mx0 = {case['nx']}
 my0 = {case['ny']}
 mz0 = {case['nz']}
 <algorithm>
-highorder = 0
+highorder = {case.get('highorder', 0)}
 Corr = {p['corr']}
 ntimes = {p['ntimes']}
 <boundaries>
@@ -114,7 +118,8 @@ def run_history(case, prec):
     import subprocess
     cfg, pi, pe = initial_state(case)
     dt = np.float32 if prec == "f32" else np.float64
-    exe = os.path.join(REF_DIR, f"tristan_ref_{case['dim']}d_dd{case['order']}_{prec}")
+    suffix = "_vay" if case.get("pusher") == "vay" else ""
+    exe = os.path.join(REF_DIR, f"tristan_ref_{case['dim']}d_dd{case['order']}_{prec}{suffix}")
     maxptl = 2 * (pi.shape[1] + pe.shape[1]) + 1000
     mx, my, mz, _, _ = orc.extent(cfg)
     with tempfile.TemporaryDirectory() as td:

@@ -7,7 +7,9 @@ The same seeded counter-streaming pair plasma (Weibel) runs in the library
 for 1000 steps (2D: order 2 on 64x48 cells, order 1 on 256x256 cells) or 500
 steps (3D, order 3; also at 12 particles per cell per species for 300
 steps, where each cell tile holds more particles than the push's deposit
-records and the multi-pass push runs).  Round-off differences grow in a
+records and the multi-pass push runs; and over 600 steps with the 4th-order
+field solver, and with the Vay pusher against the reference built with
+-Dvay).  Round-off differences grow in a
 many-particle system,
 so the bar is set by the reference against itself: the library's deviation
 from the FP64 reference may be at most a small multiple of the reference
@@ -37,7 +39,8 @@ def _gpu_history(tmpg, case, p, pi, pe):
     import orc
     dim, order = case["dim"], case["order"]
     ctx = make_context(tmpg, dim, order, case["nx"], case["ny"], case["nz"], p, ntimes=p["ntimes"],
-                       track_ids=False)
+                       track_ids=False, highorder=case.get("highorder", 0),
+                       pusher=1 if case.get("pusher") == "vay" else 0)
     cfg = orc.make_cfg(dim, order, case["nx"], case["ny"], case["nz"])
     mx, my, mz, _, _ = orc.extent(cfg)
     ctx.set_fields(np.zeros((6, mz, my, mx), np.float32))
@@ -57,7 +60,8 @@ def _with_total(h):
     return np.concatenate([h[:, 1:5], h[:, 1:5].sum(1, keepdims=True)], axis=1)
 
 
-@pytest.mark.parametrize("name", ["weibel2d_ord2", "weibel3d_ord3", "weibel2d_ord1_256", "weibel3d_ord3_dense"])
+@pytest.mark.parametrize("name", ["weibel2d_ord2", "weibel3d_ord3", "weibel2d_ord1_256", "weibel3d_ord3_dense",
+                                  "weibel2d_ord2_highorder", "weibel2d_ord1_vay"])
 def test_energy_history_tracks_reference(name):
     import tmpg
     import make_energy_history as meh
