@@ -36,6 +36,9 @@ CASES = {
     "weibel2d_ord2_highorder": dict(dim=2, order=2, nx=64, ny=48, nz=1, ppc=8, steps=600, seed=75, highorder=1),
     # 2D order 1 with the Vay pusher (the reference built with -Dvay)
     "weibel2d_ord1_vay": dict(dim=2, order=1, nx=64, ny=48, nz=1, ppc=8, steps=600, seed=76, pusher="vay"),
+    # the same two in 3D, order 3, 16^3 cells, 4 ppc per species
+    "weibel3d_ord3_highorder": dict(dim=3, order=3, nx=16, ny=16, nz=16, ppc=4, steps=300, seed=77, highorder=1),
+    "weibel3d_ord3_vay": dict(dim=3, order=3, nx=16, ny=16, nz=16, ppc=4, steps=300, seed=78, pusher="vay"),
 }
 PHYS = dict(c=0.45, corr=1.025, ntimes=4, c_omp=5.0, gamma0=0.5, me=1.0, mi=1.0, gb=0.5, theta=2.5e-3)
 EVERY = 10

@@ -61,7 +61,8 @@ def _with_total(h):
 
 
 @pytest.mark.parametrize("name", ["weibel2d_ord2", "weibel3d_ord3", "weibel2d_ord1_256", "weibel3d_ord3_dense",
-                                  "weibel2d_ord2_highorder", "weibel2d_ord1_vay"])
+                                  "weibel2d_ord2_highorder", "weibel2d_ord1_vay", "weibel3d_ord3_highorder",
+                                  "weibel3d_ord3_vay"])
 def test_energy_history_tracks_reference(name):
     import tmpg
     import make_energy_history as meh
