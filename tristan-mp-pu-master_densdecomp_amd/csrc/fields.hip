@@ -123,16 +123,25 @@ template <> struct Binom<4> {
                                    56 / 256.f, 28 / 256.f, 8 / 256.f,  1 / 256.f};
 };
 
+// the three current components of one sweep in one launch: blockIdx.z
+// runs over nzb z-blocks of component 0, then of 1, then of 2
+struct FilterIO {
+    const float* in[3];
+    float* out[3];
+};
+
 template <int DIM, int NP>
-__global__ void __launch_bounds__(FT_X * FT_TY) k_filter_fused(const float* __restrict__ in, float* __restrict__ out,
-                                                               int nx, int ny, int nz)
+__global__ void __launch_bounds__(FT_X * FT_TY) k_filter_fused(FilterIO io, int nx, int ny, int nz, int nzb)
 {
     constexpr int T = 2 * NP + 1, RX = FT_X + 2 * NP, RY = FT_Y + 2 * NP, ROWS = FT_Y / FT_TY;
     __shared__ float raw[RY][RX];
     __shared__ float xs[RY][FT_X];
+    const int comp = blockIdx.z / nzb, bz = blockIdx.z - comp * nzb;
+    const float* __restrict__ in = io.in[comp];
+    float* __restrict__ out = io.out[comp];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FT_X + tx;
     const int i0 = blockIdx.x * FT_X, j0 = blockIdx.y * FT_Y;
-    const int k0 = (DIM == 3) ? blockIdx.z * FT_Z : 0;
+    const int k0 = (DIM == 3) ? bz * FT_Z : 0;
     const int k1 = (DIM == 3) ? min(k0 + FT_Z, nz) : 1;
     const int i = i0 + tx;
     float ring[ROWS][T] = {};
@@ -384,23 +393,25 @@ int fields_filter(tmpg_ctx* ctx, int npasses)
     while (left > 0) {
         const int np = left >= 4 ? 4 : left;
         left -= np;
-        const dim3 grid((g.nx + FT_X - 1) / FT_X, (g.ny + FT_Y - 1) / FT_Y, g.dim == 3 ? (g.nz + FT_Z - 1) / FT_Z : 1);
+        const int nzb = g.dim == 3 ? (g.nz + FT_Z - 1) / FT_Z : 1;
+        const dim3 grid((g.nx + FT_X - 1) / FT_X, (g.ny + FT_Y - 1) / FT_Y, 3 * nzb);
+        FilterIO io;
         for (int q = 0; q < 3; ++q) {
-            float* a = ctx->f[JX + q];
-#define TMPG_FF(D, N) hipLaunchKernelGGL((k_filter_fused<D, N>), grid, dim3(FT_X, FT_TY), 0, ctx->stream, a, \
-                                         ctx->tmp, g.nx, g.ny, g.nz)
-            if (g.dim == 3) {
-                switch (np) { case 1: TMPG_FF(3, 1); break; case 2: TMPG_FF(3, 2); break;
-                              case 3: TMPG_FF(3, 3); break; default: TMPG_FF(3, 4); }
-            } else {
-                switch (np) { case 1: TMPG_FF(2, 1); break; case 2: TMPG_FF(2, 2); break;
-                              case 3: TMPG_FF(2, 3); break; default: TMPG_FF(2, 4); }
-            }
-#undef TMPG_FF
-            // ping-pong: the filtered array becomes the current, the old one the scratch
-            ctx->f[JX + q] = ctx->tmp;
-            ctx->tmp = a;
+            io.in[q] = ctx->f[JX + q];
+            io.out[q] = ctx->tmp[q];
         }
+#define TMPG_FF(D, N) hipLaunchKernelGGL((k_filter_fused<D, N>), grid, dim3(FT_X, FT_TY), 0, ctx->stream, io, \
+                                         g.nx, g.ny, g.nz, nzb)
+        if (g.dim == 3) {
+            switch (np) { case 1: TMPG_FF(3, 1); break; case 2: TMPG_FF(3, 2); break;
+                          case 3: TMPG_FF(3, 3); break; default: TMPG_FF(3, 4); }
+        } else {
+            switch (np) { case 1: TMPG_FF(2, 1); break; case 2: TMPG_FF(2, 2); break;
+                          case 3: TMPG_FF(2, 3); break; default: TMPG_FF(2, 4); }
+        }
+#undef TMPG_FF
+        // ping-pong: the filtered arrays become the currents, the old ones the scratch
+        for (int q = 0; q < 3; ++q) std::swap(ctx->f[JX + q], ctx->tmp[q]);
     }
     TMPG_HIP_CHECK(ctx, hipGetLastError());
     return TMPG_OK;

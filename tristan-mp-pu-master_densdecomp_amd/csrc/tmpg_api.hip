@@ -198,7 +198,7 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
     }
     int rc = 0;
     for (int q = 0; q < NF; ++q) rc |= dalloc(ctx, &ctx->f[q], ctx->g.n);
-    rc |= dalloc(ctx, &ctx->tmp, ctx->g.n);
+    for (int q = 0; q < 3; ++q) rc |= dalloc(ctx, &ctx->tmp[q], ctx->g.n);
     if (cfg->dim == 3)
         for (int q = 0; q < 6; ++q) rc |= dalloc(ctx, &ctx->prim[q], ctx->g.n);
     rc |= dalloc(ctx, &ctx->red, 16);
@@ -235,7 +235,7 @@ int tmpg_destroy(tmpg_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& p : ctx->f) dfree(p);
     for (auto& p : ctx->prim) dfree(p);
-    dfree(ctx->tmp);
+    for (auto& p : ctx->tmp) dfree(p);
     dfree(ctx->rho);
     dfree(ctx->rho_prev);
     dfree(ctx->red);

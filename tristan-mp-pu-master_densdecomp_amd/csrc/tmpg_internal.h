@@ -94,7 +94,7 @@ struct tmpg_ctx {
     hipStream_t stream = nullptr;
     float* f[tmpg::NF] = {};
     float* prim[6] = {};   // 3D primal-averaged E,B for the gather
-    float* tmp = nullptr;  // filter scratch (one component)
+    float* tmp[3] = {nullptr, nullptr, nullptr};  // filter scratch, one grid per current component
     tmpg::Species sp[2];
     bool move_pending = false;
     bool use_cell_sums = true;  // 3D: per-cell register sums of the deposit
