@@ -57,7 +57,9 @@ typedef struct tmpg_config {
     float   q[2];         /* charge per macro-particle: qi, qe */
     float   mass[2];      /* mi, me after normalisation (energy diagnostics) */
     int64_t capacity[2];  /* particle capacity per species (0: sized at first load) */
-    int32_t tile[3];      /* cell tile for the sorted push (0: default) */
+    int32_t tile[3];      /* cell tile for the sorted push (0: default, 8x8x4 in
+                             3D; 16x16 in 2D, 16x8 on grids too small for two
+                             16x16 tiles per CU) */
     int32_t sort_every;   /* tmpg_step sorts each tile's particles by cell every
                              this many steps (0: every 4, negative: never) */
     int32_t highorder;    /* nonzero: 4th-order field solver (<algorithm>

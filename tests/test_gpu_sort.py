@@ -88,7 +88,8 @@ def test_sort_orders_each_tile_by_cell(dim, shape, ppc):
     nx, ny, nz = shape
     ncell = nx * ny * (nz if dim == 3 else 1)
     f, pi, pe = random_state(dim, 2, nx, ny, nz, ppc * ncell, seed=3 + ppc)
-    ctx = make_context(tmpg, dim, 2, nx, ny, nz, PHYS, sort_every=-1)
+    tile = (8, 8, 4) if dim == 3 else (16, 16, 1)
+    ctx = make_context(tmpg, dim, 2, nx, ny, nz, PHYS, sort_every=-1, tile=tile)
     ctx.set_particles(0, pi)
     ctx.set_particles(1, pe)
     before = [ctx.get_particles(s, with_ids=True) for s in range(2)]
@@ -98,7 +99,6 @@ def test_sort_orders_each_tile_by_cell(dim, shape, ppc):
     cfg = orc.make_cfg(dim, 2, nx, ny, nz)
     _, _, _, ng, ngz = orc.extent(cfg)
     lo = (ng // 2 + 1, ng // 2 + 1, ngz // 2 + 1)
-    tile = (8, 8, 4) if dim == 3 else (16, 16, 1)
     for s in range(2):
         (p0, i0), (p1, i1) = before[s], after[s]
         assert np.array_equal(sorted_by_id(p0, i0), sorted_by_id(p1, i1))

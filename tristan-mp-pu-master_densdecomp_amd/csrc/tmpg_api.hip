@@ -168,9 +168,22 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
     ctx->my = cfg->ny + ng;
     ctx->mz = cfg->dim == 3 ? nz + ngz : 1;
     {
-        // cell tiles of the sorted push (default 8^3 in 3D, 16^2 in 2D)
+        // cell tiles of the sorted push (default 8x8x4 in 3D; 16x16 in 2D,
+        // or 16x8 when 16x16 tiles would not give every CU two workgroups)
+        int ncu = 256;
+        {
+            int dev = 0;
+            hipDeviceProp_t prop;
+            if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+                ncu = prop.multiProcessorCount;
+            else
+                (void)hipGetLastError();
+        }
         int tx = cfg->tile[0] > 0 ? cfg->tile[0] : (cfg->dim == 3 ? 8 : 16);
         int ty = cfg->tile[1] > 0 ? cfg->tile[1] : (cfg->dim == 3 ? 8 : 16);
+        if (cfg->dim == 2 && cfg->tile[1] <= 0 &&
+            (int64_t)((cfg->nx + 15) / 16) * ((cfg->ny + 15) / 16) < 2 * (int64_t)ncu)
+            ty = 8;
         int tz = cfg->dim == 3 ? (cfg->tile[2] > 0 ? cfg->tile[2] : 4) : 1;
         tx = std::min(tx, cfg->nx);
         ty = std::min(ty, cfg->ny);
