@@ -89,3 +89,88 @@ def test_fortran_host_passes_solver_and_pusher_choices(tmp_path):
         if ho:
             # the 4th-order solver changes the fields well beyond round-off
             assert not np.allclose(a[:, 1:3], base[:, 1:3], rtol=1e-6, atol=0)
+
+
+RESTART = """<restart>
+irestart = {irestart}
+intrestart = {intrestart}
+laprestart = {laprestart}
+lnamedrestartint = {named}
+"""
+
+
+def run_in(d, last, irestart=0, intrestart=2000, laprestart=0, named=1000000):
+    """the host in directory d (its restart/ folder lives there), ids tracked"""
+    d.mkdir(parents=True, exist_ok=True)
+    inp = d / "input"
+    inp.write_text(INPUT.format(last=last, fused=1, highorder=0, pusher=0) + "track_ids = 1\n"
+                   + RESTART.format(irestart=irestart, intrestart=intrestart, laprestart=laprestart, named=named))
+    out = d / "energy.txt"
+    r = subprocess.run([EXE, "-i", str(inp), "-o", str(out)], capture_output=True, text=True, timeout=120, cwd=d)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    return np.loadtxt(out, ndmin=2), r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+def test_fortran_host_checkpoint_and_resume(tmp_path):
+    """write_restart / restart() (code/output.F90:2085-2262,
+    code/restart.F90:209-254) in the reference's file format: a 40-lap run
+    that checkpoints every 20 laps, and a run resumed from its lap-20 files,
+    reach the same state at lap 40, up to the order of the deposit's FP32
+    atomics, which differs between any two runs (the bars are those of the
+    fused/split comparison above)."""
+    import shutil
+
+    import restart_io as rio
+    a_dir, b_dir = tmp_path / "a", tmp_path / "b"
+    a, out_a = run_in(a_dir, 40, intrestart=20, named=20)
+    assert "wrote restart at lap 20" in out_a and "wrote restart at lap 40" in out_a
+    names = {lap: (a_dir / f"restart/restflds.default.lap{lap:06d}.000.d",
+                   a_dir / f"restart/restprtl.default.lap{lap:06d}.000.d") for lap in (20, 40)}
+    # the lap-20 files as the reference's reader sees them
+    f20 = rio.read_fields(names[20][0])
+    assert (f20["mx"], f20["my"], f20["mz"], f20["lap"]) == (64 + 7, 48 + 7, 1, 20)
+    p20 = rio.read_particles(names[20][1])
+    n_ion, n_lec = p20["ions"], p20["lecs"]
+    assert n_ion == n_lec == 64 * 48 * 16 // 2 and p20["totalpartnum"] == n_ion + n_lec
+    for s in p20["species"]:
+        assert np.unique(s["ind"]).size == s["ind"].size
+        assert np.all(s["ch"] > 0) and np.all(np.isfinite(s["u"]))
+    # the unnamed file holds the last checkpoint
+    assert rio.read_fields(a_dir / "restart/restflds.default.000.d")["lap"] == 40
+
+    (b_dir / "restart").mkdir(parents=True)
+    for p in names[20]:
+        shutil.copy(p, b_dir / "restart" / p.name)
+    b, out_b = run_in(b_dir, 40, irestart=1, intrestart=20, laprestart=20, named=20)
+    assert "read restart: lap 20" in out_b
+    assert list(b[:, 0]) == [30, 40] and list(a[:, 0]) == [10, 20, 30, 40]
+    np.testing.assert_allclose(b[:, 3:], a[2:, 3:], rtol=1e-5)
+    np.testing.assert_allclose(b[:, 1:3], a[2:, 1:3], rtol=2e-2, atol=1e-12 * a[0, 3])
+
+    fa, fb = rio.read_fields(names[40][0]), rio.read_fields(b_dir / "restart/restflds.default.lap000040.000.d")
+    for k in rio.FIELD_NAMES:
+        scale = np.abs(fa[k]).max()
+        assert np.abs(fa[k] - fb[k]).max() <= 2e-2 * scale + 1e-30, k
+    pa, pb = rio.read_particles(names[40][1]), rio.read_particles(b_dir / "restart/restprtl.default.lap000040.000.d")
+    for sa, sb in zip(pa["species"], pb["species"]):
+        ia, ib = np.argsort(sa["ind"]), np.argsort(sb["ind"])
+        assert np.array_equal(sa["ind"][ia], sb["ind"][ib])
+        for k in ("x", "y", "z"):
+            assert np.abs(sa[k][ia] - sb[k][ib]).max() < 1e-3, k
+        for k in ("u", "v", "w"):
+            assert np.abs(sa[k][ia] - sb[k][ib]).max() < 1e-3 * np.abs(sa[k]).max(), k
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+def test_fortran_host_write_restart_file_trigger(tmp_path):
+    """a file named write_restart asks for a checkpoint at the next lap and
+    is deleted (code/output.F90:2099-2110, 2262-2265)"""
+    import restart_io as rio
+    d = tmp_path / "c"
+    d.mkdir()
+    (d / "write_restart").write_text("")
+    run_in(d, 3)
+    assert not (d / "write_restart").exists()
+    assert rio.read_fields(d / "restart/restflds.default.000.d")["lap"] == 1

@@ -66,7 +66,7 @@ def build_fortran():
     os.makedirs(BINDIR, exist_ok=True)
     fobj = os.path.join(OBJ, "fortran")
     os.makedirs(fobj, exist_ok=True)
-    srcs = ["m_tmpg.F90", "m_host_setup.F90", "tristan_gpu.F90"]
+    srcs = ["m_tmpg.F90", "m_restart_io.F90", "m_host_setup.F90", "tristan_gpu.F90"]
     objs = []
     for s in srcs:
         o = os.path.join(fobj, s.replace(".F90", ".o"))

@@ -9,6 +9,8 @@
 !   <particles> ppc0, c_omp, gamma0, delgam, me, mi
 !                                        code/particles.F90:189-256
 !   <problem> temperature_ratio          user/user_weibel.F90:82-120
+!   <restart> irestart, intrestart, laprestart, lnamedrestartint
+!                                        code/restart.F90:95-98
 ! The setup is the counter-streaming Weibel problem of
 ! user/user_weibel.F90:252-292: two plasma regions drifting -x and +x, each
 ! holding both species.
@@ -16,18 +18,20 @@
 module m_host_setup
 	use, intrinsic :: iso_c_binding
 	use m_tmpg
+	use m_restart_io
 	implicit none
 
 	type host_params
 		integer :: dim = 2, order = 3
 		integer :: nx = 100, ny = 100, nz = 1
 		integer :: last = 100, ntimes = 4, highorder = 1, interval = 10
-		integer :: fused = 1, seed = 0, pusher = 0
+		integer :: fused = 1, seed = 0, pusher = 0, track_ids = 0
 		real(8) :: c = 0.45d0, corr = 1.025d0
 		real(8) :: ppc0 = 1d0, c_omp = 0d0, gamma0 = 1d0, delgam = 1d-8, me = 1d0, mi = 20d0
 		real(8) :: temperature_ratio = 1d0
 		! normalised
 		real(8) :: qe, qi, qme, qmi, mass_e, mass_i, beta, g0
+		type(restart_params) :: rst
 	end type host_params
 
 contains
@@ -62,6 +66,9 @@ contains
 		call gi("gpu", "fused", 1, hp%fused)
 		call gi("gpu", "seed", 0, hp%seed)
 		call gi("gpu", "pusher", 0, hp%pusher)
+		call gi("gpu", "track_ids", 0, hp%track_ids)
+		call read_restart_input(inp, hp%rst)
+		hp%rst%track_ids = hp%track_ids /= 0
 		call tmpg_check(tmpg_input_close(inp), "tmpg_input_close", c_null_ptr)
 		if (hp%dim == 2) hp%nz = 1
 		call normalise(hp)
@@ -109,7 +116,7 @@ contains
 		cfg%nz = hp%nz
 		cfg%ntimes = hp%ntimes
 		cfg%compat = TMPG_COMPAT_DEFAULT
-		cfg%track_ids = 0
+		cfg%track_ids = merge(1, 0, hp%track_ids /= 0)
 		cfg%c = real(hp%c, c_float)
 		cfg%corr = real(hp%corr, c_float)
 		cfg%qm = [real(hp%qmi, c_float), real(hp%qme, c_float)]

@@ -65,6 +65,24 @@ module m_tmpg
 			integer(c_int), value :: species
 			integer(c_int64_t), intent(out) :: n
 		end function
+		! p(:) of one species as SoA; ch and id may be c_null_ptr
+		integer(c_int) function tmpg_set_particles(ctx, species, n, x, y, z, u, v, w, ch, id) &
+				bind(C, name="tmpg_set_particles")
+			import :: c_int, c_ptr, c_int64_t
+			type(c_ptr), value :: ctx
+			integer(c_int), value :: species
+			integer(c_int64_t), value :: n
+			type(c_ptr), value :: x, y, z, u, v, w, ch, id
+		end function
+		integer(c_int) function tmpg_get_particles(ctx, species, n_capacity, n, x, y, z, u, v, w, ch, id) &
+				bind(C, name="tmpg_get_particles")
+			import :: c_int, c_ptr, c_int64_t
+			type(c_ptr), value :: ctx
+			integer(c_int), value :: species
+			integer(c_int64_t), value :: n_capacity
+			integer(c_int64_t), intent(out) :: n
+			type(c_ptr), value :: x, y, z, u, v, w, ch, id
+		end function
 		integer(c_int) function tmpg_sort_particles(ctx) bind(C, name="tmpg_sort_particles")
 			import :: c_int, c_ptr
 			type(c_ptr), value :: ctx

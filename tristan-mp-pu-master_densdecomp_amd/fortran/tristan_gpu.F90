@@ -15,12 +15,13 @@ program tristan_gpu
 	use, intrinsic :: iso_c_binding
 	use m_tmpg
 	use m_host_setup
+	use m_restart_io
 	implicit none
 	type(host_params) :: hp
 	type(tmpg_config) :: cfg
 	type(c_ptr) :: ctx
 	character(len=512) :: infile, outfile, arg
-	integer :: lap, k, nargs, u
+	integer :: lap, k, nargs, u, lapst
 	real(c_double) :: d(8), ms(6)
 	integer(c_int64_t) :: ni, ne
 	real(8) :: t0, t1, tacc
@@ -47,7 +48,14 @@ program tristan_gpu
 	call read_host_input(trim(infile), hp)
 	call make_config(hp, cfg)
 	call tmpg_check(tmpg_create(cfg, ctx), "tmpg_create", c_null_ptr)
-	call init_particle_distribution_user(hp, ctx)      ! fields start at zero (user_weibel init_EMfields)
+	lapst = 1
+	if (hp%rst%irestart == 1) then
+		! code/initialize.F90:155,165: a resume reads the state instead of loading it
+		call read_restart(hp%rst, ctx, hp%dim, hp%order, hp%nx, hp%ny, hp%nz, lap)
+		lapst = lap + 1
+	else
+		call init_particle_distribution_user(hp, ctx)  ! fields start at zero (user_weibel init_EMfields)
+	endif
 	call tmpg_check(tmpg_count_particles(ctx, 0_c_int, ni), "count", ctx)
 	call tmpg_check(tmpg_count_particles(ctx, 1_c_int, ne), "count", ctx)
 	write(*, '(a,i0,a,i0,a,i0,a,i0,a,i0,a,i0)') "grid ", hp%nx, " x ", hp%ny, " x ", hp%nz, &
@@ -63,7 +71,7 @@ program tristan_gpu
 	! mainloop (code/tristanmainloop.F90:107-344)
 	call system_clock(c0, crate)
 	tacc = 0d0
-	do lap = 1, hp%last
+	do lap = lapst, hp%last
 		if (hp%fused /= 0) then
 			call tmpg_check(tmpg_step(ctx, 1_c_int), "tmpg_step", ctx)
 		else
@@ -82,6 +90,7 @@ program tristan_gpu
 			write(*, '(a,i7,4(a,es14.6))') "lap ", lap, "  E ", d(1), "  B ", d(2), "  KEi ", d(3), "  KEe ", d(4)
 			if (u /= 0) write(u, '(i7,4es24.15)') lap, d(1), d(2), d(3), d(4)
 		endif
+		call maybe_write_restart(hp%rst, ctx, lap, hp%dim, hp%order, hp%nx, hp%ny, hp%nz)
 	end do
 	call tmpg_check(tmpg_sync(ctx), "tmpg_sync", ctx)
 	call system_clock(c1)
@@ -90,7 +99,8 @@ program tristan_gpu
 	! print_timers (code/communications.F90:242-326)
 	if (ms(4) > 0d0) write(*, '(a,f10.3,a,f10.3,a,f10.3,a,f10.3)') "ms/lap: fields ", ms(1) / ms(4), &
 		"  push+deposit ", ms(2) / ms(4), "  filter ", ms(3) / ms(4), "  total ", ms(5) / ms(4)
-	write(*, '(a,es12.4,a)') "particle-steps/s ", real(ni + ne, 8) * hp%last / t1, " (host clock, incl. diagnostics)"
+	write(*, '(a,es12.4,a)') "particle-steps/s ", real(ni + ne, 8) * max(hp%last - lapst + 1, 0) / t1, &
+		" (host clock, incl. diagnostics)"
 	if (u /= 0) close(u)
 	call tmpg_check(tmpg_destroy(ctx), "tmpg_destroy", c_null_ptr)
 end program tristan_gpu
