@@ -46,6 +46,11 @@ WORKLOADS = {
                                            c=0.45, c_omp=6.0, me=1.0, mi=1.0, ntimes=4,
                                            load=[(0, 2, -1.0, 0.01), (1, 2, -1.0, 0.01),
                                                  (0, 2, 1.0, 0.01), (1, 2, 1.0, 0.01)]),
+    # the same run with the Vay pusher (the reference's -Dvay build)
+    "weibel3d_ord3_512x256x256_ppc4_vay": dict(dim=3, order=3, nx=512, ny=256, nz=256, ppc=4, gb=1.0, theta=0.01,
+                                               c=0.45, c_omp=6.0, me=1.0, mi=1.0, ntimes=4, pusher=1,
+                                               load=[(0, 2, -1.0, 0.01), (1, 2, -1.0, 0.01),
+                                                     (0, 2, 1.0, 0.01), (1, 2, 1.0, 0.01)]),
     # 3D electron-ion two-stream, 2nd order: electron beams +-0.2c over cold ions
     "twostream3d_ord2_192cube_ppc16": dict(dim=3, order=2, nx=192, ny=192, nz=192, ppc=16, gb=0.2041, theta=1e-4,
                                            c=0.45, c_omp=8.0, me=1.0, mi=16.0, ntimes=4,
@@ -103,7 +108,7 @@ def make_context(w, sort_every=0):
     cap = ncell * w["ppc"]
     ctx = tmpg.Context(w["dim"], w["order"], w["nx"], w["ny"], w["nz"], ntimes=w["ntimes"], c=w["c"],
                        qm=(nm["qmi"], nm["qme"]), q=(nm["qi"], nm["qe"]), mass=(nm["mi"], nm["me"]),
-                       capacity=(cap, cap), sort_every=sort_every)
+                       capacity=(cap, cap), sort_every=sort_every, pusher=w.get("pusher", 0))
     for sp, ppc, gb, theta in w["load"]:
         ctx.load_uniform(sp, ppc, gb, theta, 7 + 2 * sp + (gb > 0))
     if w.get("denoise"):
@@ -265,7 +270,7 @@ def main():
             print(f"cpu baseline failed: {e}", file=sys.stderr)
     out = {
         "metric": f"particle-steps/s ({w['dim']}D ord-{w['order']} full step"
-                  f"{', learned denoiser' if w.get('denoise') else ''})",
+                  f"{', learned denoiser' if w.get('denoise') else ''}{', Vay pusher' if w.get('pusher') else ''})",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,

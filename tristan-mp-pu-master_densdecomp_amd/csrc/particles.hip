@@ -113,8 +113,8 @@ __device__ __forceinline__ int64_t emit(const SpecArgs& s, const PackInfo& pk, i
 constexpr int PUSH_GATHER2 = 1, PUSH_VAY = 2;
 
 // field gather and momentum push of p; returns the displacement.
-// VAY_OK = false compiles the Boris push only (the 3D cell-sum kernel, whose
-// register budget is tight; the host sends Vay runs to the other kernels)
+// VAY_OK = false compiles the Boris push only (the Boris instantiations of
+// the 3D cell-sum kernel, whose register budget is tight)
 template <int DIM, int ORD, bool VAY_OK = true, class Src>
 __device__ __forceinline__ void push_one(const Src& src, int flags, float c, float cinv, float qm, int si, int sj,
                                          int sk, Part& p, float& dx, float& dy, float& dz)
@@ -493,7 +493,7 @@ __device__ __forceinline__ void cell_sums(int c, int r0, int n, int tx, int ty, 
     }
 }
 
-template <int ORD, bool HAS_CH, bool HAS_ID, bool MULTI>
+template <int ORD, bool HAS_CH, bool HAS_ID, bool MULTI, bool VAY>
 __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
 {
     extern __shared__ __align__(16) unsigned char smb[];
@@ -638,7 +638,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         if (active) {
             li = p.i - i0; lj = p.j - j0; lk = p.k - k0;
             ox = p.ox; oy = p.oy; oz = p.oz;
-            push_one<3, ORD, false>(src, a.flags, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx,
+            push_one<3, ORD, VAY>(src, a.flags, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx,
                                     dy, dz);
             advance(p, 3, a.nx, a.ny, a.nz, dx, dy, dz);
             q = sp.q * p.ch;
@@ -1377,7 +1377,8 @@ int particles_push(tmpg_ctx* ctx, int mode)
         const int ncell_tile = tl.tx * tl.ty * tl.tz;
         const CellsLds CL = cells_lds(a.FX, a.FY, a.FZ, a.JX, a.JY, a.JZ, ncell_tile);
         const bool cells = g.dim == 3 && ctx->use_cell_sums && ncell_tile <= CELLS_T / 2 * 2 &&
-                           CL.bytes <= 160 * 1024 && ctx->cfg.pusher == 0;
+                           CL.bytes <= 160 * 1024;
+        const bool vay = ctx->cfg.pusher == 1;
         // order 3 only: there a direct deposit (about 300 LDS atomics per
         // particle) costs more than draining the records in passes; at
         // order 2 the direct deposits of the excess particles are cheaper
@@ -1390,9 +1391,11 @@ int particles_push(tmpg_ctx* ctx, int mode)
             if (Dv == 3 && cells) {
                 // tiles that hold more particles than deposit records on
                 // average get the kernel that drains its records in passes
-                auto k = k_push_cells<Ov, Cv, Iv, false>;
+                // Vay runs get their own instantiations, so that the Boris
+                // ones keep their register budget
+                auto k = vay ? k_push_cells<Ov, Cv, Iv, false, true> : k_push_cells<Ov, Cv, Iv, false, false>;
                 if constexpr (Ov == 3)
-                    if (dense) k = k_push_cells<Ov, Cv, Iv, true>;
+                    if (dense) k = vay ? k_push_cells<Ov, Cv, Iv, true, true> : k_push_cells<Ov, Cv, Iv, true, false>;
                 (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)CL.bytes);
                 hipLaunchKernelGGL(k, dim3((unsigned)tl.ntiles), dim3(CELLS_T), CL.bytes, ctx->stream, a, CL);
             } else {
