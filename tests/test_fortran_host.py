@@ -8,6 +8,8 @@ import subprocess
 import numpy as np
 import pytest
 
+from gpu_util import periodic_diff
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tristan-mp-pu-master_densdecomp_amd", "bin", "tristan-mp-gpu")
 
@@ -174,3 +176,83 @@ def test_fortran_host_write_restart_file_trigger(tmp_path):
     run_in(d, 3)
     assert not (d / "write_restart").exists()
     assert rio.read_fields(d / "restart/restflds.default.000.d")["lap"] == 1
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+@pytest.mark.parametrize("dim,order", [(2, 2), (3, 3)])
+def test_fortran_host_resumes_reference_state(tmp_path, dim, order):
+    """Fixture exchange through the restart format: the reference's state
+    before a step (tests/golden/ref_<dim>d_ord<order>.npz, from the compiled
+    reference) is written as restflds/restprtl files, the Fortran host
+    resumes from them, runs one lap and checkpoints; that checkpoint matches
+    the reference FP64 build's state after the step."""
+    import ast
+
+    import restart_io as rio
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"ref_{dim}d_ord{order}.npz"))
+    prm = ast.literal_eval(str(g["params"]))
+    nx, ny, nz = prm["nx"], prm["ny"], prm["nz"]
+    ng = 5 if order == 1 else 7
+    h = ng // 2
+    pad = ((h, ng - h) if dim == 3 else (0, 0), (h, ng - h), (h, ng - h))
+    fields = {k: np.pad(g["f_in"][q], pad, mode="wrap") for q, k in enumerate(rio.FIELD_NAMES)}
+    d = tmp_path / "r"
+    (d / "restart").mkdir(parents=True)
+    rio.write_fields(d / "restart/restflds.default.000.d", fields, lap=0)
+    sp = []
+    for key in ("pi_in", "pe_in"):
+        p = g[key]
+        sp.append(dict(zip(rio.PART_REALS, p)) | dict(ind=np.arange(p.shape[1], dtype=np.int32)))
+    rio.write_particles(d / "restart/restprtl.default.000.d", sp[0], sp[1])
+    inp = d / "input"
+    inp.write_text(f"""<time>
+last = 1
+c = {prm['c']}
+<grid>
+mx0 = {nx}
+my0 = {ny}
+mz0 = {nz}
+<algorithm>
+highorder = 0
+Corr = {prm['corr']}
+ntimes = {prm['ntimes']}
+<output>
+interval = 1
+<particles>
+ppc0 = {prm['ppc0']}
+me = {prm['me']}
+mi = {prm['mi']}
+gamma0 = {prm['gamma0']}
+c_omp = {prm['c_omp']}
+<gpu>
+dim = {dim}
+order = {order}
+track_ids = 1
+<restart>
+irestart = 1
+intrestart = 1
+""")
+    r = subprocess.run([EXE, "-i", str(inp)], capture_output=True, text=True, timeout=120, cwd=d)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert "read restart: lap 0" in r.stdout and "wrote restart at lap 1" in r.stdout
+    f = rio.read_fields(d / "restart/restflds.default.000.d")
+    assert f["lap"] == 1
+    inner = (slice(h, h + nz) if dim == 3 else slice(0, 1), slice(h, h + ny), slice(h, h + nx))
+    ref = g["step_f64"]
+    for q, k in enumerate(rio.FIELD_NAMES):
+        scale = np.abs(ref[q]).max()
+        err = np.abs(f[k][inner] - ref[q]).max()
+        assert err <= 2e-6 * scale, (k, err, scale)
+    pr = rio.read_particles(d / "restart/restprtl.default.000.d")
+    for s, key in enumerate(("step_pi_f64", "step_pe_f64")):
+        want = g[key]
+        got = pr["species"][s]
+        order_ = np.argsort(got["ind"])
+        assert np.array_equal(got["ind"][order_], np.arange(want.shape[1]))
+        # positions modulo the periodic box; in 2D the library keeps no z
+        # (the reference's 2D z drifts with w and is never read)
+        for q, (k, n) in enumerate(zip(("x", "y", "z")[:dim], (nx, ny, nz))):
+            assert np.abs(periodic_diff(got[k][order_], want[q], h + 1, n)).max() < 2e-5, k
+        for q, k in enumerate(("u", "v", "w"), start=3):
+            assert np.abs(got[k][order_] - want[q]).max() <= 2e-6 * np.abs(want[q]).max(), k
