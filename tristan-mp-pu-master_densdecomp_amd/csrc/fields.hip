@@ -146,12 +146,33 @@ __global__ void __launch_bounds__(FT_X * FT_TY) k_filter_fused(FilterIO io, int 
     const int i = i0 + tx;
     float ring[ROWS][T] = {};
     const int kfirst = (DIM == 3) ? k0 - NP : 0, klast = (DIM == 3) ? k1 - 1 + NP : 0;
-    for (int kk = kfirst; kk <= klast; ++kk) {
+    // each thread's share of a staged plane, loaded one plane ahead into
+    // registers so that the loads of plane kk+1 are in flight while plane kk
+    // is filtered
+    constexpr int NTH = FT_X * FT_TY, NL = (RY * RX + NTH - 1) / NTH;
+    float pre[NL];
+    auto fetch = [&](int kk) {
         const int64_t plane = (int64_t)nx * ny * ((DIM == 3) ? wrapany(kk, nz) : 0);
-        for (int e = tid; e < RY * RX; e += FT_X * FT_TY) {
-            const int r = e / RX, c = e - r * RX;
-            raw[r][c] = in[plane + wrapany(i0 + c - NP, nx) + (int64_t)nx * wrapany(j0 + r - NP, ny)];
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int e = tid + u * NTH;
+            if (e < RY * RX) {
+                const int r = e / RX, c = e - r * RX;
+                pre[u] = in[plane + wrapany(i0 + c - NP, nx) + (int64_t)nx * wrapany(j0 + r - NP, ny)];
+            }
         }
+    };
+    fetch(kfirst);
+    for (int kk = kfirst; kk <= klast; ++kk) {
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int e = tid + u * NTH;
+            if (e < RY * RX) {
+                const int r = e / RX, c = e - r * RX;
+                raw[r][c] = pre[u];
+            }
+        }
+        if (kk < klast) fetch(kk + 1);
         __syncthreads();
         for (int r = ty; r < RY; r += FT_TY) {
             float v = 0.0f;
