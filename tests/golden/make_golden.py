@@ -132,8 +132,8 @@ def read_state(path, cfg, dtype):
     return g, out
 
 
-def run_ref(dim, order, prec, inp, state, mode, cfg, ni, ne, maxhlf, suffix=""):
-    exe = os.path.join(REF_DIR, f"tristan_ref_{dim}d_dd{order}_{prec}{suffix}")
+def run_ref(dim, order, prec, inp, state, mode, cfg, ni, ne, maxhlf, suffix="", exe=None):
+    exe = exe or os.path.join(REF_DIR, f"tristan_ref_{dim}d_dd{order}_{prec}{suffix}")
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "input")
         open(fin, "w").write(inp)
@@ -321,6 +321,44 @@ def main_filter2():
         print(out, os.path.getsize(out))
 
 
+def main_zzag():
+    """ref_{dim}d_zzag.npz: the reference built with -Dzzag (the original
+    trilinear mover, code/particles_movedeposit.F90:98-347, and the zigzag
+    deposit, code/particles.F90:550-669): one main-loop step and the mover
+    alone (driver mode 1), 2D and 3D; nghost = 5 as for order 1"""
+    for dim in (2, 3):
+        case = CASES[dim]
+        cfg = orc.make_cfg(dim, 1, case["nx"], case["ny"], case["nz"], ntimes=PHYS["ntimes"])
+        rng = np.random.default_rng(9100 + dim)
+        nz = case["nz"] if dim == 3 else 1
+        f_in = rng.uniform(-0.05, 0.05, (6, nz, case["ny"], case["nx"])).astype(np.float32)
+        pi = make_particles(rng, cfg, case["npart"])
+        pe = make_particles(rng, cfg, case["npart"] + 17)
+        maxptl = 20000
+        maxhlf = maxptl // 2
+        inp = input_text(dim, case["nx"], case["ny"], case["nz"], maxptl)
+        fix = dict(f_in=f_in, pi_in=pi, pe_in=pe)
+        sl = orc.interior_slices(cfg)
+        with tempfile.TemporaryDirectory() as td:
+            for prec, dt in (("f32", np.float32), ("f64", np.float64)):
+                exe = os.path.join(REF_DIR, f"tristan_ref_{dim}d_{prec}_zzag")
+                f6 = orc.fill_from_interior(cfg, f_in.astype(dt), dt)
+                st = os.path.join(td, f"state_{prec}.bin")
+                write_state(st, f6, pi, pe, dt)
+                g, qi_, qe_ = run_ref(dim, 1, prec, inp, st, 0, cfg, pi.shape[1], pe.shape[1], maxhlf, exe=exe)
+                fix[f"step_{prec}"] = g[(slice(None),) + sl]
+                fix[f"step_pi_{prec}"] = qi_
+                fix[f"step_pe_{prec}"] = qe_
+                _, qi_, qe_ = run_ref(dim, 1, prec, inp, st, 1, cfg, pi.shape[1], pe.shape[1], maxhlf, exe=exe)
+                fix[f"move_pi_{prec}"] = qi_
+                fix[f"move_pe_{prec}"] = qe_
+        params = dict(dim=dim, order="zzag", nx=case["nx"], ny=case["ny"], nz=nz, **PHYS)
+        fix["params"] = np.array(repr(params))
+        out = os.path.join(HERE, f"ref_{dim}d_zzag.npz")
+        np.savez_compressed(out, **fix)
+        print(out, os.path.getsize(out))
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["filter2"]:
         main_filter2()
@@ -328,5 +366,7 @@ if __name__ == "__main__":
         main_highorder()
     elif sys.argv[1:] == ["vay"]:
         main_vay()
+    elif sys.argv[1:] == ["zzag"]:
+        main_zzag()
     else:
         main()
