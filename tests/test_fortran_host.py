@@ -176,6 +176,10 @@ def test_fortran_host_write_restart_file_trigger(tmp_path):
     run_in(d, 3)
     assert not (d / "write_restart").exists()
     assert rio.read_fields(d / "restart/restflds.default.000.d")["lap"] == 1
+    # intrestart = 0 turns the periodic checkpoints off
+    e = tmp_path / "e"
+    run_in(e, 3, intrestart=0, named=0)
+    assert not (e / "restart").exists()
 
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")

@@ -106,10 +106,12 @@ contains
 		integer, intent(in) :: lap, dim, order, nx, ny, nz
 		logical :: exst
 		integer :: u
+		! an interval of 0 or less turns the periodic checkpoints off
 		inquire(file="write_restart", exist=exst)
-		if (.not. (mod(lap, rp%intrestart) == 0 .or. exst)) return
+		if (.not. (exst .or. (rp%intrestart > 0 .and. mod(lap, max(rp%intrestart, 1)) == 0))) return
 		call write_restart_files(rp, ctx, lap, dim, order, nx, ny, nz, 0)
-		if (mod(lap, rp%namedrestartint) == 0) call write_restart_files(rp, ctx, lap, dim, order, nx, ny, nz, lap)
+		if (rp%namedrestartint > 0 .and. mod(lap, max(rp%namedrestartint, 1)) == 0) &
+			call write_restart_files(rp, ctx, lap, dim, order, nx, ny, nz, lap)
 		if (exst) then
 			open(newunit=u, file="write_restart", form="unformatted")
 			close(u, status="delete")
@@ -150,6 +152,11 @@ contains
 		do s = 1, 2
 			call tmpg_check(tmpg_count_particles(ctx, int(s - 1, c_int), cnt(s)), "write_restart: count", ctx)
 		end do
+		! the record's counts are default integers (code/output.F90:2212)
+		if (max(cnt(1), cnt(2)) > huge(ions) / 2) then
+			write(*, '(a,2(1x,i0))') "write_restart: particle counts beyond the format's integer range:", cnt
+			stop 2
+		endif
 		ions = int(cnt(1))
 		lecs = int(cnt(2))
 		maxhlf = max(ions, lecs)
