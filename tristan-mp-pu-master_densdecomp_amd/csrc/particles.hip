@@ -115,6 +115,18 @@ constexpr int PUSH_GATHER2 = 1, PUSH_VAY = 2;
 // field gather and momentum push of p; returns the displacement.
 // VAY_OK = false compiles the Boris push only (the Boris instantiations of
 // the 3D cell-sum kernel, whose register budget is tight)
+template <bool VAY_OK = true>
+__device__ __forceinline__ void push_fields(const float* E, int flags, float c, float cinv, float qm, Part& p,
+                                            float& dx, float& dy, float& dz)
+{
+    const float se = 0.5f * qm, sb = 0.5f * qm * cinv;
+    if (VAY_OK && (flags & PUSH_VAY))
+        vay(c, cinv, se * E[0], se * E[1], se * E[2], sb * E[3], sb * E[4], sb * E[5], p.u, p.v, p.w, dx, dy, dz);
+    else
+        boris(c, cinv, se * E[0], se * E[1], se * E[2], sb * E[3], sb * E[4], sb * E[5], p.u, p.v, p.w, dx, dy,
+              dz);
+}
+
 template <int DIM, int ORD, bool VAY_OK = true, class Src>
 __device__ __forceinline__ void push_one(const Src& src, int flags, float c, float cinv, float qm, int si, int sj,
                                          int sk, Part& p, float& dx, float& dy, float& dz)
@@ -493,9 +505,17 @@ __device__ __forceinline__ void cell_sums(int c, int r0, int n, int tx, int ty, 
     }
 }
 
-template <int ORD, bool HAS_CH, bool HAS_ID, bool MULTI, bool VAY>
+// The tile extents TX x TY x TZ are compile-time constants (the default
+// 8x8x4 tile): the staged-field and current strides are then constants too,
+// so the 64 nodes of a gather and the sums of a cell address LDS with one
+// base register plus immediate offsets, and cell <-> tile coordinates are
+// shifts instead of integer divisions.
+template <int ORD, bool HAS_CH, bool HAS_ID, bool MULTI, bool VAY, int TX, int TY, int TZ>
 __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
 {
+    constexpr int FX = TX + 3, FY = TY + 3, FZ = TZ + 3;   // staged fields: nodes -1 .. T+1
+    constexpr int JX = TX + 5, JY = TY + 5, JZ = TZ + 5;   // tile currents: nodes -2 .. T+2
+    constexpr int fplane = FX * FY * FZ, jplane = JX * JY * JZ, nc = TX * TY * TZ;
     extern __shared__ __align__(16) unsigned char smb[];
     double* sj = reinterpret_cast<double*>(smb + L.j_off);
     float* sf = reinterpret_cast<float*>(smb + L.f_off);
@@ -519,17 +539,17 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     const int ti = (int)(t % tl.ntx);
     const int tj = (int)((t / tl.ntx) % tl.nty);
     const int tk = (int)(t / ((int64_t)tl.ntx * tl.nty));
-    const int i0 = ti * tl.tx, j0 = tj * tl.ty, k0 = tk * tl.tz;
-    const int fplane = L.fplane, jplane = L.jplane, nc = L.nc;
-    const int tid = threadIdx.x, NT = blockDim.x;
+    const int i0 = ti * TX, j0 = tj * TY, k0 = tk * TZ;
+    const int tid = threadIdx.x;
+    constexpr int NT = CELLS_T;
     for (int e = tid; e < fplane; e += NT) {
-        const int li = e % a.FX;
-        const int r = e / a.FX;
-        const int lj = r % a.FY;
-        const int lk = r / a.FY;
-        const int gi = wrap1(i0 - a.fo + li, a.nx);
-        const int gj = wrap1(j0 - a.fo + lj, a.ny);
-        const int gk = wrap1(k0 - a.fo + lk, a.nz);
+        const int li = e % FX;
+        const int r = e / FX;
+        const int lj = r % FY;
+        const int lk = r / FY;
+        const int gi = wrap1(i0 - 1 + li, a.nx);
+        const int gj = wrap1(j0 - 1 + lj, a.ny);
+        const int gk = wrap1(k0 - 1 + lk, a.nz);
         const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
         float2* d = reinterpret_cast<float2*>(sf) + 3 * e;   // node-major
         d[0] = make_float2(a.f6[0][g], a.f6[1][g]);
@@ -542,10 +562,13 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     if (tid == 0) *nrec = 0u;
     __syncthreads();
 
-    const LdsSrc6 src{sf, a.FX, a.FY};
-    const LdsSink snk{sj, a.JX, a.JY, jplane};
+    const LdsSink snk{sj, JX, JY, jplane};
     const int lane = __lane_id();
     const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // tile of a cell: the tile extents are powers of two in the default tile
+    auto tile_at = [&](int i, int j, int k) -> int64_t {
+        return (int64_t)(i / TX) + (int64_t)tl.ntx * ((j / TY) + (int64_t)tl.nty * (k / TZ));
+    };
 
     // ---- the records so far: crossers deposited directly, the rest
     // counting-sorted by cell and summed per cell (phases B and C); the
@@ -557,7 +580,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         for (int k = tid; k < nd; k += NT) {
             const int r = CELLS_R - 1 - k;
             const int c = rc[r];
-            const int li = c % tl.tx, lj = (c / tl.tx) % tl.ty, lk = c / (tl.tx * tl.ty);
+            const int li = c % TX, lj = (c / TX) % TY, lk = c / (TX * TY);
             deposit<3, ORD, LdsSink, false>(snk, li + 2, lj + 2, lk + 2, rec[r], rec[CELLS_R + r],
                                             rec[2 * CELLS_R + r], rec[3 * CELLS_R + r], rec[4 * CELLS_R + r],
                                             rec[5 * CELLS_R + r], rq[r]);
@@ -611,8 +634,8 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         // the half is uniform per wave, so each half is its own code path
         for (int wk = tid; wk < 2 * nc; wk += NT) {
             const int c = perm[wk % nc];
-            if (wk < nc) cell_sums<ORD, 0>(c, start[c], cnt[c], tl.tx, tl.ty, idx, rec, rq, snk);
-            else cell_sums<ORD, 1>(c, start[c], cnt[c], tl.tx, tl.ty, idx, rec, rq, snk);
+            if (wk < nc) cell_sums<ORD, 0>(c, start[c], cnt[c], TX, TY, idx, rec, rq, snk);
+            else cell_sums<ORD, 1>(c, start[c], cnt[c], TX, TY, idx, rec, rq, snk);
         }
         __syncthreads();
         for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
@@ -638,8 +661,12 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         if (active) {
             li = p.i - i0; lj = p.j - j0; lk = p.k - k0;
             ox = p.ox; oy = p.oy; oz = p.oz;
-            push_one<3, ORD, VAY>(src, a.flags, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo, lk + a.fo, p, dx,
-                                    dy, dz);
+            {
+                float E[6];
+                gather3_lds<ORD, FX, FY>(sf, (a.flags & PUSH_GATHER2) != 0, li + 1, lj + 1, lk + 1, p.ox, p.oy,
+                                         p.oz, E);
+                push_fields<VAY>(E, a.flags, a.c, a.cinv, sp.qm, p, dx, dy, dz);
+            }
             advance(p, 3, a.nx, a.ny, a.nz, dx, dy, dz);
             q = sp.q * p.ch;
             fits = fits4<ORD>(oy, dy) && fits4<ORD>(oz, dz);
@@ -664,7 +691,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
                          : CELLS_R - 1 - ((int)(old >> 16) + __popcll(mc & ltmask));
         }
         if (active) {
-            const int c = li + tl.tx * (lj + tl.ty * lk);
+            const int c = li + TX * (lj + TY * lk);
             if (r >= 0) {
                 rec[0 * CELLS_R + r] = ox;
                 rec[1 * CELLS_R + r] = oy;
@@ -679,7 +706,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
                 // record space exhausted: deposit this particle at once
                 deposit<3, ORD>(snk, li + 2, lj + 2, lk + 2, ox, oy, oz, dx, dy, dz, q);
             }
-            td = tile_of(tl, p.i, p.j, p.k);
+            td = tile_at(p.i, p.j, p.k);
         }
         // slots in b: leavers reserve one each in their new tile, the
         // wave's stayers one block in this tile; both atomics are in
@@ -747,10 +774,10 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     }
     }
     for (int e = tid; e < jplane; e += NT) {
-        const int li = e % a.JX;
-        const int r = e / a.JX;
-        const int lj = r % a.JY;
-        const int lk = r / a.JY;
+        const int li = e % JX;
+        const int r = e / JX;
+        const int lj = r % JY;
+        const int lk = r / JY;
         const int gi = wrap1(i0 - 2 + li, a.nx);
         const int gj = wrap1(j0 - 2 + lj, a.ny);
         const int gk = wrap1(k0 - 2 + lk, a.nz);
@@ -1376,8 +1403,12 @@ int particles_push(tmpg_ctx* ctx, int mode)
         }
         const int ncell_tile = tl.tx * tl.ty * tl.tz;
         const CellsLds CL = cells_lds(a.FX, a.FY, a.FZ, a.JX, a.JY, a.JZ, ncell_tile);
-        const bool cells = g.dim == 3 && ctx->use_cell_sums && ncell_tile <= CELLS_T / 2 * 2 &&
+        // the cell-sum kernel is compiled for the default 8x8x4 tile; other
+        // tilings (explicit, or clamped on grids under 8x8x4 cells) take the
+        // general tiled kernel
+        const bool cells = g.dim == 3 && ctx->use_cell_sums && tl.tx == 8 && tl.ty == 8 && tl.tz == 4 &&
                            CL.bytes <= 160 * 1024;
+        (void)ncell_tile;
         const bool vay = ctx->cfg.pusher == 1;
         // order 3 only: there a direct deposit (about 300 LDS atomics per
         // particle) costs more than draining the records in passes; at
@@ -1393,9 +1424,12 @@ int particles_push(tmpg_ctx* ctx, int mode)
                 // average get the kernel that drains its records in passes
                 // Vay runs get their own instantiations, so that the Boris
                 // ones keep their register budget
-                auto k = vay ? k_push_cells<Ov, Cv, Iv, false, true> : k_push_cells<Ov, Cv, Iv, false, false>;
+                auto k = vay ? k_push_cells<Ov, Cv, Iv, false, true, 8, 8, 4>
+                             : k_push_cells<Ov, Cv, Iv, false, false, 8, 8, 4>;
                 if constexpr (Ov == 3)
-                    if (dense) k = vay ? k_push_cells<Ov, Cv, Iv, true, true> : k_push_cells<Ov, Cv, Iv, true, false>;
+                    if (dense)
+                        k = vay ? k_push_cells<Ov, Cv, Iv, true, true, 8, 8, 4>
+                                : k_push_cells<Ov, Cv, Iv, true, false, 8, 8, 4>;
                 (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)CL.bytes);
                 hipLaunchKernelGGL(k, dim3((unsigned)tl.ntiles), dim3(CELLS_T), CL.bytes, ctx->stream, a, CL);
             } else {

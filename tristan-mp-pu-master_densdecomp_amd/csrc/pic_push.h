@@ -150,6 +150,74 @@ __device__ __forceinline__ void gather(const Src& src, bool quirk2, int i, int j
     }
 }
 
+// The 3D gather from node-major primal fields in LDS whose strides FX, FY
+// (nodes per row, rows per plane) are compile-time constants: one base
+// address for the particle's 4x4x4 node block, every node at a constant
+// offset from it (LDS reads with immediate offsets, no per-node address
+// arithmetic).  (i,j,k) is the particle's cell in staged-node coordinates,
+// at least 1 on every axis.  Same arithmetic as gather<3, ORD>.
+template <int ORD, int FX, int FY>
+__device__ __forceinline__ void gather3_lds(const float* __restrict__ s, bool quirk2, int i, int j, int k, float ox,
+                                            float oy, float oz, float* E)
+{
+    float Spx[4], Sdx[4], Spy[4], Sdy[4], Spz[4], Sdz[4];
+    int dix, diy, diz;
+    gather_weights<ORD>(ox, quirk2, Spx, Sdx, dix);
+    gather_weights<ORD>(oy, quirk2, Spy, Sdy, diy);
+    gather_weights<ORD>(oz, quirk2, Spz, Sdz, diz);
+    constexpr int N0 = GatherRange<ORD>::N0, N1 = GatherRange<ORD>::N1, NB = N1 - N0;
+    constexpr int NR = NB * NB;          // node rows (y, z) of the block
+    constexpr int RB = 2;                // rows per batch of reads
+    constexpr int NBATCH = NR / RB;
+    constexpr int BW = RB * NB * 3;      // float2 per batch
+#pragma unroll
+    for (int q = 0; q < 6; ++q) E[q] = 0.0f;
+    const float2* p = reinterpret_cast<const float2*>(s) + 3 * ((i - 1) + FX * ((j - 1) + FY * (k - 1)));
+    // the reads of batch n + 1 are issued before batch n is summed, so a
+    // wave waits for LDS once per batch instead of once per node
+    float2 buf[2][BW];
+    auto load = [&](int n, float2* d) {
+#pragma unroll
+        for (int rr = 0; rr < RB; ++rr) {
+            const int r = n * RB + rr;
+            const int b = N0 + r % NB, c = N0 + r / NB;
+            const float2* row = p + 3 * FX * (b + FY * c);
+#pragma unroll
+            for (int t = 0; t < NB; ++t)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) d[(rr * NB + t) * 3 + m] = row[3 * (N0 + t) + m];
+        }
+    };
+    load(0, buf[0]);
+#pragma unroll
+    for (int n = 0; n < NBATCH; ++n) {
+        if (n + 1 < NBATCH) load(n + 1, buf[(n + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+        const float2* d = buf[n & 1];
+#pragma unroll
+        for (int rr = 0; rr < RB; ++rr) {
+            const int r = n * RB + rr;
+            const int b = N0 + r % NB, c = N0 + r / NB;
+            float acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                const float w = Spx[N0 + t];
+                const float2 a0 = d[(rr * NB + t) * 3], a1 = d[(rr * NB + t) * 3 + 1], a2 = d[(rr * NB + t) * 3 + 2];
+                acc[0] += a0.x * w;
+                acc[1] += a0.y * w;
+                acc[2] += a1.x * w;
+                acc[3] += a1.y * w;
+                acc[4] += a2.x * w;
+                acc[5] += a2.y * w;
+            }
+            const float wyz = Spy[b] * Spz[c];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) E[q] += acc[q] * wyz;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // Esirkepov deposit of one particle from old cell (i,j,k) / offsets o*, with
 // displacement d*, charge q.  The stored current follows the reference's
 // sign (E += cur): x current at node a = q * sum_{a'<=a} dSx(a') *
