@@ -13,3 +13,11 @@ for p in (HERE, os.path.join(PKG, "python")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="module")
+def tmpg():
+    """the ctypes binding with libtmpg.so loaded (raises when it is missing)"""
+    import tmpg as t
+    t.load_library()
+    return t

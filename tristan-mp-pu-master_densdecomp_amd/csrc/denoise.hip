@@ -533,6 +533,9 @@ struct tmpg_model {
     double* dloss = nullptr;
     double* dots = nullptr;              // per image <d,t>, <t,t> (amplitude term)
     float lam = 0.0f;                    // weight of the amplitude term
+    // ordering between a context's stream and the model's stream in
+    // denoise_apply (created once with the model)
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
     // device-resident training set
     std::vector<float*> set_lo, set_t;
     int set_nx = 0, set_ny = 0, set_nz = 0;
@@ -818,19 +821,29 @@ int tmpg_model_create(int dim, uint64_t seed, tmpg_model** out)
         for (int co = 0; co < L.cout; ++co)
             for (int t = 0; t < taps; ++t)
                 for (int ci = 0; ci < cr; ++ci) hw[(size_t)co * L.kpad + t * L.cin + ci] = (float)(a * rnd());
-        std::vector<float> zero((size_t)nw, 0.0f);
-        (void)hipMemcpy(L.w, hw.data(), nw * sizeof(float), hipMemcpyHostToDevice);
-        (void)hipMemset(L.b, 0, L.cout * sizeof(float));
-        (void)hipMemset(L.m1w, 0, nw * sizeof(float));
-        (void)hipMemset(L.m2w, 0, nw * sizeof(float));
-        (void)hipMemset(L.m1b, 0, L.cout * sizeof(float));
-        (void)hipMemset(L.m2b, 0, L.cout * sizeof(float));
         m->L.push_back(L);
+        // weights, biases and Adam moments: a failed copy is an error, not
+        // a silently uninitialised model
+        hipError_t e = hipMemcpy(L.w, hw.data(), nw * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(L.b, 0, L.cout * sizeof(float));
+        if (e == hipSuccess) e = hipMemset(L.m1w, 0, nw * sizeof(float));
+        if (e == hipSuccess) e = hipMemset(L.m2w, 0, nw * sizeof(float));
+        if (e == hipSuccess) e = hipMemset(L.m1b, 0, L.cout * sizeof(float));
+        if (e == hipSuccess) e = hipMemset(L.m2b, 0, L.cout * sizeof(float));
+        if (e != hipSuccess) {
+            tmpg_model_destroy(m);
+            return TMPG_EHIP;
+        }
     }
     rc |= dn_alloc(m, &m->dloss, 1);
     rc |= dn_alloc(m, &m->dots, 2 * kMaxBatch);
     if (rc || prep_weights(m)) { tmpg_model_destroy(m); return TMPG_ENOMEM; }
-    (void)hipStreamSynchronize(m->stream);
+    if (hipEventCreateWithFlags(&m->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev_out, hipEventDisableTiming) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess) {
+        tmpg_model_destroy(m);
+        return TMPG_EHIP;
+    }
     *out = m;
     return TMPG_OK;
 }
@@ -846,6 +859,8 @@ int tmpg_model_destroy(tmpg_model* m)
     for (auto p : m->set_t) f(p);
     f(m->dloss);
     f(m->dots);
+    if (m->ev_in) (void)hipEventDestroy(m->ev_in);
+    if (m->ev_out) (void)hipEventDestroy(m->ev_out);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return TMPG_OK;
@@ -1052,25 +1067,36 @@ int denoise_apply(tmpg_ctx* ctx, tmpg_model* m)
     int rc = set_geometry(m, g, ctx->g.nx, ctx->g.ny, ctx->g.nz, 1);
     if (rc) { ctx->err = m->err; return rc; }
     if (m->scale <= 0.0f) { ctx->err = "denoiser scale not set"; return TMPG_ESTATE; }
-    // the model's stream waits for the context's stream and vice versa
-    hipEvent_t e1, e2;
-    TMPG_HIP_CHECK(ctx, hipEventCreateWithFlags(&e1, hipEventDisableTiming));
-    TMPG_HIP_CHECK(ctx, hipEventCreateWithFlags(&e2, hipEventDisableTiming));
-    TMPG_HIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
-    TMPG_HIP_CHECK(ctx, hipStreamWaitEvent(m->stream, e1, 0));
+    // the model's stream waits for the context's stream and vice versa (the
+    // model's two events; a failure still joins the streams, so no work of
+    // this call is left running behind the caller)
+    TMPG_HIP_CHECK(ctx, hipEventRecord(m->ev_in, ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipStreamWaitEvent(m->stream, m->ev_in, 0));
     const int64_t n = g.npix;
     hipLaunchKernelGGL(k_pack_input3, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, ctx->f[JX],
                        ctx->f[JY], ctx->f[JZ], n, 1.0f / m->scale, m->x0);
     rc = forward(m, g, false);
-    if (rc) { ctx->err = m->err; return rc; }
-    hipLaunchKernelGGL(k_apply, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, n, m->curl, m->scale,
-                       ctx->f[JX], ctx->f[JY], ctx->f[JZ]);
-    TMPG_HIP_CHECK(ctx, hipGetLastError());
-    TMPG_HIP_CHECK(ctx, hipEventRecord(e2, m->stream));
-    TMPG_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, e2, 0));
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
-    return TMPG_OK;
+    if (rc) {
+        ctx->err = m->err;
+    } else {
+        hipLaunchKernelGGL(k_apply, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, m->stream, n, m->curl, m->scale,
+                           ctx->f[JX], ctx->f[JY], ctx->f[JZ]);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            ctx->err = std::string("denoiser launch: ") + hipGetErrorString(e);
+            rc = TMPG_EHIP;
+        }
+    }
+    const hipError_t e1 = hipEventRecord(m->ev_out, m->stream);
+    const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(ctx->stream, m->ev_out, 0) : e1;
+    if (e2 != hipSuccess) {
+        (void)hipStreamSynchronize(m->stream);
+        if (!rc) {
+            ctx->err = std::string("denoiser stream join: ") + hipGetErrorString(e2);
+            rc = TMPG_EHIP;
+        }
+    }
+    return rc;
 }
 }  // namespace tmpg
 

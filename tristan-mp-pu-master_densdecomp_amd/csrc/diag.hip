@@ -356,20 +356,25 @@ int particles_load_uniform(tmpg_ctx* ctx, int s, int ppc, float drift, float the
     rc = alloc_pbuf(ctx, flat, std::max<int64_t>(n0 + count, 1), false, ids);
     if (rc) { free_pbuf(old); return rc; }
     const size_t fb = (size_t)n0 * sizeof(float), ib = (size_t)n0 * sizeof(int32_t);
+    hipError_t e = hipSuccess;
     if (n0 > 0) {
-        hipMemcpyAsync(flat.cell, old.cell, ib, hipMemcpyDeviceToDevice, ctx->stream);
-        hipMemcpyAsync(flat.ox, old.ox, fb, hipMemcpyDeviceToDevice, ctx->stream);
-        hipMemcpyAsync(flat.oy, old.oy, fb, hipMemcpyDeviceToDevice, ctx->stream);
-        if (g.dim == 3) hipMemcpyAsync(flat.oz, old.oz, fb, hipMemcpyDeviceToDevice, ctx->stream);
-        hipMemcpyAsync(flat.u, old.u, fb, hipMemcpyDeviceToDevice, ctx->stream);
-        hipMemcpyAsync(flat.v, old.v, fb, hipMemcpyDeviceToDevice, ctx->stream);
-        hipMemcpyAsync(flat.w, old.w, fb, hipMemcpyDeviceToDevice, ctx->stream);
-        if (ids) hipMemcpyAsync(flat.id, old.id, ib, hipMemcpyDeviceToDevice, ctx->stream);
+        // the existing particles move to the front of the new buffer
+        auto cp = [&](void* d, const void* s, size_t bytes) {
+            if (e == hipSuccess) e = hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+        };
+        cp(flat.cell, old.cell, ib);
+        cp(flat.ox, old.ox, fb);
+        cp(flat.oy, old.oy, fb);
+        if (g.dim == 3) cp(flat.oz, old.oz, fb);
+        cp(flat.u, old.u, fb);
+        cp(flat.v, old.v, fb);
+        cp(flat.w, old.w, fb);
+        if (ids) cp(flat.id, old.id, ib);
     }
     const int ntab = 4096;
     std::vector<float> tab = mj_inverse_cdf(theta, ntab);
     float* dtab = nullptr;
-    hipError_t e = hipMalloc(&dtab, ntab * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dtab, ntab * sizeof(float));
     if (e == hipSuccess) e = hipMemcpyAsync(dtab, tab.data(), ntab * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && count > 0) {
         dim3 grid((unsigned)((count + 255) / 256));
@@ -379,7 +384,8 @@ int particles_load_uniform(tmpg_ctx* ctx, int s, int ppc, float drift, float the
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (dtab) hipFree(dtab);
+    else (void)hipStreamSynchronize(ctx->stream);
+    if (dtab) (void)hipFree(dtab);
     free_pbuf(old);
     if (e != hipSuccess) { free_pbuf(flat); ctx->err = hipGetErrorString(e); return TMPG_EHIP; }
     rc = particles_bin(ctx, s, flat, n0 + count, false, ids);

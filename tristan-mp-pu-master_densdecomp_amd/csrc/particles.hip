@@ -408,11 +408,20 @@ __device__ __forceinline__ void cell_sums(int c, int r0, int n, int tx, int ty, 
     for (int u = 0; u < 6; ++u)
 #pragma unroll
         for (int b = 0; b < 4; ++b) { az[u][b][0] = 0.0f; az[u][b][1] = 0.0f; }
-    for (int rr = r0; rr < r1; ++rr) {
+    // the next record is read while this one is summed
+    float nr[7];
+    auto fetch = [&](int rr) {
         const int r = idx[rr];
-        const float ox = rec[r], oy = rec[CELLS_R + r], oz = rec[2 * CELLS_R + r];
-        const float dx = rec[3 * CELLS_R + r], dy = rec[4 * CELLS_R + r], dz = rec[5 * CELLS_R + r];
-        const float q = rq[r];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) nr[m] = rec[m * CELLS_R + r];
+        nr[6] = rq[r];
+    };
+    fetch(r0);
+    for (int rr = r0; rr < r1; ++rr) {
+        const float ox = nr[0], oy = nr[1], oz = nr[2];
+        const float dx = nr[3], dy = nr[4], dz = nr[5];
+        const float q = nr[6];
+        if (rr + 1 < r1) fetch(rr + 1);
         float dSx[6], Ax[6], dSy[4], Ay[4], dSz[4], Az[4];
         {
             float S6[6];
@@ -574,16 +583,18 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     // counting-sorted by cell and summed per cell (phases B and C); the
     // record space is then empty again
     auto drain = [&]() {
-        // ---- direct deposit of the y/z crossers
+        // ---- direct deposit of the y/z crossers, one z node plane of one
+        // crosser per thread (so every wave shares the work)
         const int nr = (int)(*nrec & 0xffffu);
         const int nd = (int)(*nrec >> 16);
-        for (int k = tid; k < nd; k += NT) {
+        constexpr int WZ = Win<ORD>::W;
+        for (int w = tid; w < nd * WZ; w += NT) {
+            const int k = w / WZ, zc = w - k * WZ;
             const int r = CELLS_R - 1 - k;
             const int c = rc[r];
             const int li = c % TX, lj = (c / TX) % TY, lk = c / (TX * TY);
-            deposit<3, ORD, LdsSink, false>(snk, li + 2, lj + 2, lk + 2, rec[r], rec[CELLS_R + r],
-                                            rec[2 * CELLS_R + r], rec[3 * CELLS_R + r], rec[4 * CELLS_R + r],
-                                            rec[5 * CELLS_R + r], rq[r]);
+            deposit3_plane<ORD>(snk, li + 2, lj + 2, lk + 2, rec[r], rec[CELLS_R + r], rec[2 * CELLS_R + r],
+                                rec[3 * CELLS_R + r], rec[4 * CELLS_R + r], rec[5 * CELLS_R + r], rq[r], zc);
         }
 
         // ---- phase B: counting sort of the cell-summed records by cell (the
@@ -670,7 +681,18 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             advance(p, 3, a.nx, a.ny, a.nz, dx, dy, dz);
             q = sp.q * p.ch;
             fits = fits4<ORD>(oy, dy) && fits4<ORD>(oz, dz);
+            td = tile_at(p.i, p.j, p.k);
         }
+        // slots in b: leavers reserve one each in their new tile, the
+        // wave's stayers one block in this tile.  Both global atomics are
+        // issued here and their results used only after the deposit record
+        // is written, so the record work covers their latency.
+        const bool stay = active && td == t;
+        const uint64_t m = __ballot(stay);
+        int32_t lslot = 0, b0 = 0;
+        if (active && !stay) lslot = atomicAdd(sp.cnt_b + td, 1);
+        const int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
+        if (m && lane == leader) b0 = atomicAdd(sp.cnt_b + t, __popcll(m));
         // record slots, one LDS atomic per wave: cell-summed records grow
         // from 0 up, y/z crossers from the top down.  A wave whose block
         // would make the two meet takes its block back and deposits at
@@ -706,17 +728,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
                 // record space exhausted: deposit this particle at once
                 deposit<3, ORD>(snk, li + 2, lj + 2, lk + 2, ox, oy, oz, dx, dy, dz, q);
             }
-            td = tile_at(p.i, p.j, p.k);
         }
-        // slots in b: leavers reserve one each in their new tile, the
-        // wave's stayers one block in this tile; both atomics are in
-        // flight together (one round trip)
-        const bool stay = active && td == t;
-        const uint64_t m = __ballot(stay);
-        int32_t lslot = 0, b0 = 0;
-        if (active && !stay) lslot = atomicAdd(sp.cnt_b + td, 1);
-        const int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
-        if (m && lane == leader) b0 = atomicAdd(sp.cnt_b + t, __popcll(m));
         b0 = __shfl(b0, leader, 64);
         const int32_t slot = stay ? b0 + __popcll(m & ltmask) : lslot;
         if (active) emit<3, HAS_CH, HAS_ID>(sp, a.pk, slots, td, slot, p);
@@ -1415,7 +1427,10 @@ int particles_push(tmpg_ctx* ctx, int mode)
         // order 2 the direct deposits of the excess particles are cheaper
         const bool dense = ctx->cfg.order == 3 &&
                            ctx->sp[0].n + ctx->sp[1].n > (int64_t)CELLS_R * 9 / 10 * tl.ntiles;
-        if (ctx->push_events) (void)hipEventRecord(ctx->push_events[0], ctx->stream);
+        if (ctx->push_events) {
+            TMPG_HIP_CHECK(ctx, hipEventRecord(ctx->push_events[0], ctx->stream));
+            ctx->push_recorded = true;
+        }
         dispatch(g.dim, ctx->cfg.order, ch, id, [&](auto D, auto O, auto C, auto I) {
             constexpr int Dv = decltype(D)::value, Ov = decltype(O)::value;
             constexpr bool Cv = decltype(C)::value, Iv = decltype(I)::value;
@@ -1439,7 +1454,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
             }
         });
         TMPG_HIP_CHECK(ctx, hipGetLastError());
-        if (ctx->push_events) (void)hipEventRecord(ctx->push_events[1], ctx->stream);
+        if (ctx->push_events) TMPG_HIP_CHECK(ctx, hipEventRecord(ctx->push_events[1], ctx->stream));
         // overflow list of the previous step
         for (int s = 0; s < 2; ++s) {
             Species& sp = ctx->sp[s];

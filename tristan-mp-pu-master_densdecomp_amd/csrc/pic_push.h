@@ -303,4 +303,60 @@ __device__ __forceinline__ void deposit(const Sink& snk, int i, int j, int k, fl
     }
 }
 
+// One z node plane (window slot zc, 0 <= zc < W) of the 3D Esirkepov deposit
+// of one particle: exactly the values deposit<3, ORD, Sink, false> adds in
+// that plane.  Splitting a particle's deposit over its W planes lets W
+// threads share it.
+template <int ORD, class Sink>
+__device__ __forceinline__ void deposit3_plane(const Sink& snk, int i, int j, int k, float ox, float oy, float oz,
+                                               float dx, float dy, float dz, float q, int zc)
+{
+    constexpr int W = Win<ORD>::W;
+    float S1x[W], dSx[W], S1y[W], dSy[W], S1z[W], dSz[W];
+    const int bx = axis_deposit<ORD>(ox, dx, S1x, dSx);
+    const int by = axis_deposit<ORD>(oy, dy, S1y, dSy);
+    const int bz = axis_deposit<ORD>(oz, dz, S1z, dSz);
+    float Ax[W], Ay[W], Cx[W], Cy[W];
+    float run_x = 0.0f, run_y = 0.0f, run_z = 0.0f;
+    float az = 0.0f, dsz = 0.0f, cz = 0.0f;
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+        Ax[t] = S1x[t] + 0.5f * dSx[t];
+        Ay[t] = S1y[t] + 0.5f * dSy[t];
+        run_x += dSx[t];
+        run_y += dSy[t];
+        run_z += dSz[t];
+        Cx[t] = q * run_x;
+        Cy[t] = q * run_y;
+        if (t == zc) {
+            az = S1z[t] + 0.5f * dSz[t];
+            dsz = dSz[t];
+            cz = q * run_z;
+        }
+    }
+    const float twelfth = 1.0f / 12.0f;
+    int cc[W];
+#pragma unroll
+    for (int t = 0; t < W; ++t) cc[t] = snk.col(i + bx + t);
+#pragma unroll
+    for (int b = 0; b < W; ++b) {
+        const auto r = snk.row(j + by + b, k + bz + zc);
+        const float wyz = Ay[b] * az + twelfth * dSy[b] * dsz;
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            if (t < W - 1) snk.add(0, r, cc[t], Cx[t] * wyz);
+            if (b < W - 1) snk.add(1, r, cc[t], Cy[b] * (Ax[t] * az + twelfth * dSx[t] * dsz));
+        }
+    }
+    // the last plane's z prefix is the telescoped zero
+    if (zc < W - 1) {
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            const auto r = snk.row(j + by + b, k + bz + zc);
+#pragma unroll
+            for (int t = 0; t < W; ++t) snk.add(2, r, cc[t], cz * (Ax[t] * Ay[b] + twelfth * dSx[t] * dSy[b]));
+        }
+    }
+}
+
 }  // namespace tmpg

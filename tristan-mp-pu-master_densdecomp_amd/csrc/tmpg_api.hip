@@ -221,9 +221,16 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
         tmpg_destroy(ctx);
         return TMPG_ENOMEM;
     }
-    for (int q = 0; q < NF; ++q) (void)hipMemsetAsync(ctx->f[q], 0, ctx->g.n * sizeof(float), ctx->stream);
-    (void)hipMemsetAsync(ctx->flags, 0, 16 * sizeof(int32_t), ctx->stream);
-    for (auto& ev : ctx->ev) (void)hipEventCreate(&ev);
+    e = hipSuccess;
+    for (int q = 0; q < NF && e == hipSuccess; ++q) e = hipMemsetAsync(ctx->f[q], 0, ctx->g.n * sizeof(float), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->flags, 0, 16 * sizeof(int32_t), ctx->stream);
+    for (auto& ev : ctx->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+        g_create_err = std::string("tmpg_create: ") + hipGetErrorString(e);
+        tmpg_destroy(ctx);
+        return TMPG_EHIP;
+    }
     for (int s = 0; s < 2; ++s) {
         // empty, binned species (the push needs valid tile counts for both)
         if (particles_bin(ctx, s, PBuf(), 0, false, cfg->track_ids != 0)) {
@@ -524,6 +531,31 @@ int tmpg_add_current(tmpg_ctx* ctx)
     return fields_add_current(ctx);
 }
 
+// elapsed times of the phases of the tmpg_step recorded in ring slot `slot`
+// (its events complete), added to ctx->prof_sum
+static int prof_fold(tmpg_ctx* ctx, int slot)
+{
+    hipEvent_t* ev = &ctx->prof[(size_t)slot * 8];
+    TMPG_HIP_CHECK(ctx, hipEventSynchronize(ev[5]));
+    float t[6] = {0, 0, 0, 0, 0, 0};
+    TMPG_HIP_CHECK(ctx, hipEventElapsedTime(&t[0], ev[0], ev[1]));
+    TMPG_HIP_CHECK(ctx, hipEventElapsedTime(&t[1], ev[1], ev[2]));
+    TMPG_HIP_CHECK(ctx, hipEventElapsedTime(&t[2], ev[2], ev[3]));
+    TMPG_HIP_CHECK(ctx, hipEventElapsedTime(&t[3], ev[3], ev[4]));
+    TMPG_HIP_CHECK(ctx, hipEventElapsedTime(&t[4], ev[4], ev[5]));
+    // the push kernel's events are recorded only when it ran (no particles:
+    // none)
+    if (ctx->prof_push[(size_t)slot]) TMPG_HIP_CHECK(ctx, hipEventElapsedTime(&t[5], ev[6], ev[7]));
+    double* ms = ctx->prof_sum;
+    ms[0] += (double)t[0] + t[2] + t[4];
+    ms[1] += t[1];
+    ms[2] += t[3];
+    ms[4] += (double)t[0] + t[1] + t[2] + t[3] + t[4];
+    ms[5] += t[5];
+    ctx->prof_steps++;
+    return TMPG_OK;
+}
+
 int tmpg_step(tmpg_ctx* ctx, int nsteps)
 {
     if (!valid_ctx(ctx) || nsteps < 0) return TMPG_EINVAL;
@@ -532,6 +564,10 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
         TMPG_HIP_CHECK(ctx, hipHostMalloc((void**)&ctx->hostcnt, 8 * sizeof(int32_t), hipHostMallocDefault));
         TMPG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->cnt_ev, hipEventDisableTiming));
     }
+    auto record = [&](hipEvent_t e) -> int {
+        TMPG_HIP_CHECK(ctx, hipEventRecord(e, ctx->stream));
+        return TMPG_OK;
+    };
     for (int s = 0; s < nsteps && rc == 0; ++s) {
         if (ctx->cnt_pending) {
             // overflow counters copied at an earlier step, read without a
@@ -558,37 +594,49 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps)
             rc = particles_check(ctx);
             if (rc) break;
         }
-        // per-phase HIP events on the context's stream, read by tmpg_last_timings
-        const size_t need = (size_t)(ctx->prof_steps + 1) * 8;
-        while (ctx->prof.size() < need) {
+        // per-phase HIP events on the context's stream, read by
+        // tmpg_last_timings; the ring slot of the step kProfRing steps back is
+        // folded into the running sums before it is reused
+        while (ctx->prof.size() < (size_t)tmpg_ctx::kProfRing * 8) {
             hipEvent_t e;
             TMPG_HIP_CHECK(ctx, hipEventCreate(&e));
             ctx->prof.push_back(e);
         }
-        hipEvent_t* ev = &ctx->prof[(size_t)ctx->prof_steps * 8];
-        ctx->prof_steps++;
+        ctx->prof_push.resize(tmpg_ctx::kProfRing, 0);
+        if (ctx->prof_unread == tmpg_ctx::kProfRing) {
+            rc = prof_fold(ctx, ctx->prof_head);
+            if (rc) break;
+            ctx->prof_unread--;
+        }
+        const int slot = ctx->prof_head;
+        hipEvent_t* ev = &ctx->prof[(size_t)slot * 8];
         // code/tristanmainloop.F90:117-242 with the ghost fills and the
         // current fold absorbed into periodic indexing, reset_currents moved
         // before the push and add_current fused into the E update
-        (void)hipEventRecord(ev[0], ctx->stream);
+        TMPG_HIP_CHECK(ctx, hipEventRecord(ev[0], ctx->stream));
         rc = fields_bhalf(ctx);
         if (!rc) rc = fields_zero_j(ctx);
-        (void)hipEventRecord(ev[1], ctx->stream);
+        if (!rc) rc = record(ev[1]);
         if (!rc && ctx->sort_every > 0 && ++ctx->steps_since_sort >= ctx->sort_every) {
             ctx->steps_since_sort = 0;
             rc = particles_sort_tiles(ctx);
         }
         if (!rc) rc = pre_push(ctx);
         ctx->push_events = ev + 6;   // around the tiled push kernel itself
+        ctx->push_recorded = false;
         if (!rc) rc = particles_push(ctx, 0);
         ctx->push_events = nullptr;
-        (void)hipEventRecord(ev[2], ctx->stream);
+        if (!rc) rc = record(ev[2]);
         if (!rc) rc = fields_bhalf(ctx);
-        (void)hipEventRecord(ev[3], ctx->stream);
+        if (!rc) rc = record(ev[3]);
         if (!rc) rc = ctx->denoiser ? denoise_apply(ctx, ctx->denoiser) : fields_filter(ctx, ctx->cfg.ntimes);
-        (void)hipEventRecord(ev[4], ctx->stream);
+        if (!rc) rc = record(ev[4]);
         if (!rc) rc = fields_efull(ctx, true);
-        (void)hipEventRecord(ev[5], ctx->stream);
+        if (!rc) rc = record(ev[5]);
+        if (rc) break;    // a failed step leaves its slot unclaimed
+        ctx->prof_push[(size_t)slot] = ctx->push_recorded ? 1 : 0;
+        ctx->prof_head = (slot + 1) % tmpg_ctx::kProfRing;
+        ctx->prof_unread++;
         if (!rc && !ctx->cnt_pending && (ctx->steps_since_check & 3) == 0) {
             for (int q = 0; q < 2; ++q) {
                 if (ctx->sp[q].ocnt)
@@ -607,28 +655,15 @@ int tmpg_last_timings(tmpg_ctx* ctx, double ms[6])
 {
     if (!valid_ctx(ctx) || !ms) return TMPG_EINVAL;
     TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    for (int q = 0; q < 6; ++q) ms[q] = 0.0;
-    for (int s = 0; s < ctx->prof_steps; ++s) {
-        hipEvent_t* ev = &ctx->prof[(size_t)s * 8];
-        float t[6] = {0, 0, 0, 0, 0, 0};
-        (void)hipEventElapsedTime(&t[0], ev[0], ev[1]);
-        (void)hipEventElapsedTime(&t[1], ev[1], ev[2]);
-        (void)hipEventElapsedTime(&t[2], ev[2], ev[3]);
-        (void)hipEventElapsedTime(&t[3], ev[3], ev[4]);
-        (void)hipEventElapsedTime(&t[4], ev[4], ev[5]);
-        // the push kernel's events are recorded only when it ran (no
-        // particles: none); a failed query leaves no sticky error behind
-        if (hipEventElapsedTime(&t[5], ev[6], ev[7]) != hipSuccess) {
-            t[5] = 0.0f;
-            (void)hipGetLastError();
-        }
-        ms[0] += (double)t[0] + t[2] + t[4];
-        ms[1] += t[1];
-        ms[2] += t[3];
-        ms[4] += (double)t[0] + t[1] + t[2] + t[3] + t[4];
-        ms[5] += t[5];
+    // fold the unread steps, oldest first
+    const int R = tmpg_ctx::kProfRing;
+    for (; ctx->prof_unread > 0; --ctx->prof_unread) {
+        const int rc = prof_fold(ctx, ((ctx->prof_head - ctx->prof_unread) % R + R) % R);
+        if (rc) return rc;
     }
+    for (int q = 0; q < 6; ++q) ms[q] = ctx->prof_sum[q];
     ms[3] = (double)ctx->prof_steps;
+    for (auto& v : ctx->prof_sum) v = 0.0;
     ctx->prof_steps = 0;
     return TMPG_OK;
 }

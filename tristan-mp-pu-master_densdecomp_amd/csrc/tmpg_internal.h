@@ -111,9 +111,18 @@ struct tmpg_ctx {
     bool cnt_pending = false;
     int steps_since_check = 0;
     hipEvent_t ev[8] = {};
-    std::vector<hipEvent_t> prof;  // 6 events per tmpg_step iteration since the last read
-    int prof_steps = 0;
+    // per-phase timing of tmpg_step: a ring of kProfRing steps x 8 events;
+    // a step's elapsed times are folded into prof_sum when its slot is
+    // reused or tmpg_last_timings reads them, so the event count stays fixed
+    static constexpr int kProfRing = 64;
+    std::vector<hipEvent_t> prof;  // kProfRing * 8 events, created at the first tmpg_step
+    int prof_head = 0;             // slot of the next step
+    int prof_unread = 0;           // recorded steps not yet folded into prof_sum
+    double prof_sum[6] = {};       // folded phase times (ms), tmpg_last_timings order
+    int prof_steps = 0;            // steps folded into prof_sum
+    std::vector<char> prof_push;   // per slot: the push kernel's two events were recorded
     hipEvent_t* push_events = nullptr;  // set during tmpg_step: recorded around the tiled push kernel
+    bool push_recorded = false;         // the push recorded push_events in this step
     int sort_every = 4;                 // tmpg_step sorts each tile's particles by cell every this many steps
     int steps_since_sort = 0;
     std::string err;
