@@ -142,6 +142,18 @@ int tmpg_step(tmpg_ctx* ctx, int nsteps);
  *          needs tmpg_track_continuity(ctx,1) before the step), else -1,
  * out[7] = L2 norm of rho used for that check */
 int tmpg_diag(tmpg_ctx* ctx, double out[8]);
+/* Gauss's law (no reference counterpart: the reference computes no
+ * divergence diagnostic).  rho = the charge density of tmpg_charge_density,
+ * F^n rho = rho after the n binomial passes the step applies to the
+ * deposited current (cfg.ntimes, or the denoiser's base passes), div_b E =
+ * the backward-difference divergence at the nodes of the staggered E:
+ *   out[0] = max |div_b E - F^n rho| over the interior nodes,
+ *   out[1] = rms of F^n rho, out[2] = max |F^n rho|, out[3] = max |div_b E|.
+ * The step adds F^n cur to E with div_b(cur) = rho_new - rho_old (Esirkepov)
+ * and an E update whose curl has no divergence (highorder = 0), so out[0]
+ * keeps its initial value up to round-off; a neutral start (E = 0, the
+ * species at the same places) keeps it near zero. */
+int tmpg_gauss_residual(tmpg_ctx* ctx, double out[4]);
 /* charge density at the nodes (interior nx*ny*nz, x fastest), FP64,
  * deposited with the order's own shape from the current positions */
 int tmpg_charge_density(tmpg_ctx* ctx, double* rho);

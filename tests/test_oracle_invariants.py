@@ -15,7 +15,10 @@ formulas:
   {4,2,1}/16 in 2D, 3x3x3 weights {8,4,2,1}/64 in 3D);
 * orc_mover in a pure, uniform magnetic field: |u| is conserved and u turns
   about B by 2 atan(|b|), b = qm B / (2 c gamma) (the Boris rotation,
-  code/particles_movedeposit.F90:1224-1244).
+  code/particles_movedeposit.F90:1224-1244);
+* orc_step end to end: the main loop keeps Gauss's law div_b E = F^n rho
+  (F^n the filter passes applied to the current) at round-off from a
+  neutral start.
 
 A plausible slip (a node off by one, a missing half-cell shift, a wrong sign
 of the current, a transposed filter weight, a dropped Boris factor) breaks
@@ -170,3 +173,52 @@ def test_boris_rotation_in_pure_b(dim, order):
     bmag = 0.5 * abs(qm) * np.linalg.norm(B) / (cfg.c * gamma)
     # a particle of negative q/m turns positively about B (right-handed)
     np.testing.assert_allclose(ang, -np.sign(qm) * 2.0 * np.arctan(bmag), rtol=1e-11, atol=1e-14)
+
+
+def binomial(a, npasses, dim):
+    """npasses periodic (1,2,1)/4 passes along every axis of a (z, y, x) grid"""
+    for _ in range(npasses):
+        for ax in ([2, 1, 0] if dim == 3 else [2, 1]):
+            a = 0.25 * (np.roll(a, 1, axis=ax) + np.roll(a, -1, axis=ax)) + 0.5 * a
+    return a
+
+
+def div_b(e3, dim):
+    """backward-difference divergence at the nodes of the staggered E"""
+    d = (e3[0] - np.roll(e3[0], 1, axis=2)) + (e3[1] - np.roll(e3[1], 1, axis=1))
+    if dim == 3:
+        d = d + (e3[2] - np.roll(e3[2], 1, axis=0))
+    return d
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (2, 3), (3, 1), (3, 2), (3, 3)])
+def test_step_keeps_gauss_law(dim, order):
+    """the main loop (E += filtered Esirkepov current, E += c Corr curl_b B)
+    keeps div_b E - F^n rho at its initial value: from a neutral start
+    (E = 0, both species at the same places) it stays at round-off"""
+    nx, ny, nz = 12, 10, 8
+    ntimes = 3
+    cfg = orc.make_cfg(dim, order, nx, ny, nz, ntimes=ntimes)
+    mx, my, mz, ng, ngz = orc.extent(cfg)
+    h, hz = ng // 2, ngz // 2
+    rng = np.random.default_rng(5 + dim * 3 + order)
+    n = 4 * nx * ny * (nz if dim == 3 else 1)
+    pos = np.vstack([h + 1 + rng.random(n) * nx, h + 1 + rng.random(n) * ny,
+                     (hz + 1 + rng.random(n) * nz) if dim == 3 else np.full(n, hz + 1.5)])
+    pi = np.ascontiguousarray(np.vstack([pos, rng.normal(0, 0.3, (3, n)), np.ones((1, n))]))
+    pe = np.ascontiguousarray(np.vstack([pos, rng.normal(0, 0.3, (3, n)), np.ones((1, n))]))
+    nm = orc.normalize(np.float64, 0.45, 4.0, 0.3, 8.0, 1.0, 1.0)
+    f6 = orc.zeros_grid(cfg, 6, np.float64)
+    j3 = orc.zeros_grid(cfg, 3, np.float64)
+    sl = orc.interior_slices(cfg)
+    for _ in range(2):
+        orc.step(cfg, f6, j3, pi, pe, nm, 10)
+        rho = binomial(orc.rho(cfg, pi, nm["qi"]) + orc.rho(cfg, pe, nm["qe"]), ntimes, dim)
+        e3 = np.stack([f6[(q,) + sl] for q in range(3)])
+        res = div_b(e3, dim) - rho
+        rms = np.sqrt(np.mean(rho ** 2))
+        assert rms > 1e-6 * abs(nm["qi"])          # the species have separated
+        assert np.abs(res).max() < 1e-11 * rms, np.abs(res).max() / rms
+    # against the unfiltered rho the check fails: the step adds F^n cur to E
+    raw = orc.rho(cfg, pi, nm["qi"]) + orc.rho(cfg, pe, nm["qe"])
+    assert np.abs(div_b(e3, dim) - raw).max() > 1e-3 * rms

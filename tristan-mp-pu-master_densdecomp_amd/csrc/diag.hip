@@ -185,6 +185,89 @@ __global__ void __launch_bounds__(256) k_continuity(const double* __restrict__ r
     if (threadIdx.x == 0) atomicAdd(out + 1, s2);
 }
 
+// one binomial (1,2,1)/4 pass along `axis` of a periodic FP64 grid
+__global__ void __launch_bounds__(256) k_binomial_axis(const double* __restrict__ in, double* __restrict__ out,
+                                                       int nx, int ny, int nz, int axis)
+{
+    const int64_t n = (int64_t)nx * ny * nz;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx);
+        const int64_t r = t / nx;
+        const int j = (int)(r % ny);
+        const int k = (int)(r / ny);
+        int64_t m, p;
+        if (axis == 0) {
+            m = (i == 0 ? nx - 1 : i - 1) + (int64_t)nx * (j + (int64_t)ny * k);
+            p = (i == nx - 1 ? 0 : i + 1) + (int64_t)nx * (j + (int64_t)ny * k);
+        } else if (axis == 1) {
+            m = i + (int64_t)nx * ((j == 0 ? ny - 1 : j - 1) + (int64_t)ny * k);
+            p = i + (int64_t)nx * ((j == ny - 1 ? 0 : j + 1) + (int64_t)ny * k);
+        } else {
+            m = i + (int64_t)nx * (j + (int64_t)ny * (k == 0 ? nz - 1 : k - 1));
+            p = i + (int64_t)nx * (j + (int64_t)ny * (k == nz - 1 ? 0 : k + 1));
+        }
+        out[t] = 0.25 * (in[m] + in[p]) + 0.5 * in[t];
+    }
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_down(v, o, 64);
+        v = w > v ? w : v;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double m = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = sh[k] > m ? sh[k] : m;
+    __syncthreads();
+    return m;
+}
+
+// Gauss's law on the nodes: r = div_b E - rho with the backward differences
+// of the E update's divergence (E components staggered as the Yee grid:
+// ex(i) sits at i + 1/2).  out[0] max|r|, out[1] sum rho^2, out[2] max|rho|,
+// out[3] max|div_b E| (maxima of non-negative doubles via their bit
+// patterns, which order alike)
+__global__ void __launch_bounds__(256) k_gauss(const float* __restrict__ ex, const float* __restrict__ ey,
+                                               const float* __restrict__ ez, const double* __restrict__ rho,
+                                               int dim, int nx, int ny, int nz, double* out)
+{
+    __shared__ double sh[4];
+    const int64_t n = (int64_t)nx * ny * nz;
+    double mr = 0.0, s2 = 0.0, mq = 0.0, md = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx);
+        const int64_t r = t / nx;
+        const int j = (int)(r % ny);
+        const int k = (int)(r / ny);
+        const int im = i == 0 ? nx - 1 : i - 1, jm = j == 0 ? ny - 1 : j - 1;
+        double div = ((double)ex[t] - ex[im + (int64_t)nx * (j + (int64_t)ny * k)]) +
+                     ((double)ey[t] - ey[i + (int64_t)nx * (jm + (int64_t)ny * k)]);
+        if (dim == 3) {
+            const int km = k == 0 ? nz - 1 : k - 1;
+            div += (double)ez[t] - ez[i + (int64_t)nx * (j + (int64_t)ny * km)];
+        }
+        const double q = rho[t];
+        mr = fmax(mr, fabs(div - q));
+        md = fmax(md, fabs(div));
+        mq = fmax(mq, fabs(q));
+        s2 += q * q;
+    }
+    const double vals[3] = {mr, mq, md};
+    const int slots[3] = {0, 2, 3};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double m = block_max(vals[c], sh);
+        if (threadIdx.x == 0) atomicMax((unsigned long long*)(out + slots[c]), (unsigned long long)__double_as_longlong(m));
+    }
+    s2 = block_sum<256>(s2, sh);
+    if (threadIdx.x == 0) atomicAdd(out + 1, s2);
+}
+
 // ------------------------------------------------------------------ loader
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
@@ -310,6 +393,35 @@ int diag_continuity(tmpg_ctx* ctx, double* maxres, double* rhonorm)
     TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     *maxres = h[0];
     *rhonorm = std::sqrt(h[1]);
+    return TMPG_OK;
+}
+
+int diag_gauss(tmpg_ctx* ctx, double* rho_a, double* rho_b, int npasses, double out[4])
+{
+    const Grid& g = ctx->g;
+    int rc = diag_rho(ctx, rho_a);
+    if (rc) return rc;
+    // the n binomial passes the step applies to the deposited current
+    double* a = rho_a;
+    double* b = rho_b;
+    const int blocks = (int)std::min<int64_t>((g.n + 255) / 256, 4096);
+    for (int p = 0; p < npasses; ++p)
+        for (int ax = 0; ax < g.dim; ++ax) {
+            hipLaunchKernelGGL(k_binomial_axis, dim3(blocks), dim3(256), 0, ctx->stream, a, b, g.nx, g.ny, g.nz, ax);
+            std::swap(a, b);
+        }
+    double* d = ctx->red + 8;
+    TMPG_HIP_CHECK(ctx, hipMemsetAsync(d, 0, 4 * sizeof(double), ctx->stream));
+    hipLaunchKernelGGL(k_gauss, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[EX], ctx->f[EY], ctx->f[EZ], a, g.dim,
+                       g.nx, g.ny, g.nz, d);
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    double h[4];
+    TMPG_HIP_CHECK(ctx, hipMemcpyAsync(h, d, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TMPG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    out[0] = h[0];
+    out[1] = std::sqrt(h[1] / (double)g.n);
+    out[2] = h[2];
+    out[3] = h[3];
     return TMPG_OK;
 }
 

@@ -214,7 +214,7 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
     for (int q = 0; q < 3; ++q) rc |= dalloc(ctx, &ctx->tmp[q], ctx->g.n);
     if (cfg->dim == 3)
         for (int q = 0; q < 6; ++q) rc |= dalloc(ctx, &ctx->prim[q], ctx->g.n);
-    rc |= dalloc(ctx, &ctx->red, 16);
+    rc |= dalloc(ctx, &ctx->red, 16);   // [0,4) energies, [4,6) continuity, [8,12) Gauss
     rc |= dalloc(ctx, &ctx->flags, 16);
     if (rc) {
         g_create_err = ctx->err;
@@ -258,6 +258,7 @@ int tmpg_destroy(tmpg_ctx* ctx)
     for (auto& p : ctx->tmp) dfree(p);
     dfree(ctx->rho);
     dfree(ctx->rho_prev);
+    dfree(ctx->rho_work);
     dfree(ctx->red);
     dfree(ctx->flags);
     for (auto& s : ctx->sp) free_species(s);
@@ -707,6 +708,20 @@ int tmpg_diag(tmpg_ctx* ctx, double out[8])
         if (rc) return rc;
     }
     return TMPG_OK;
+}
+
+int tmpg_gauss_residual(tmpg_ctx* ctx, double out[4])
+{
+    if (!valid_ctx(ctx) || !out) return TMPG_EINVAL;
+    int rc = ensure_rho(ctx);
+    if (!rc && !ctx->rho_work) rc = dalloc(ctx, &ctx->rho_work, ctx->g.n);
+    if (rc) return rc ? TMPG_ENOMEM : TMPG_OK;
+    int np = ctx->cfg.ntimes;
+    if (ctx->denoiser) {
+        rc = tmpg_model_get_base_passes(ctx->denoiser, &np);
+        if (rc) { ctx->err = "tmpg_gauss_residual: denoiser base passes"; return rc; }
+    }
+    return diag_gauss(ctx, ctx->rho, ctx->rho_work, np, out);
 }
 
 int tmpg_load_uniform(tmpg_ctx* ctx, int species, int ppc, float drift_gb, float theta, uint64_t seed)

@@ -102,6 +102,7 @@ struct tmpg_ctx {
     // diagnostics (FP64)
     double* rho = nullptr;
     double* rho_prev = nullptr;
+    double* rho_work = nullptr;  // second FP64 grid of the Gauss diagnostic
     double* red = nullptr;     // reduction scratch
     bool track_cont = false;
     bool have_rho_prev = false;
@@ -163,5 +164,8 @@ int particles_sort_tiles(tmpg_ctx* ctx);
 int diag_energies(tmpg_ctx* ctx, double out[4]);
 int diag_rho(tmpg_ctx* ctx, double* rho_dev);
 int diag_continuity(tmpg_ctx* ctx, double* maxres, double* rhonorm);
+// Gauss's law residual with rho filtered by npasses binomial passes; rho_a
+// and rho_b are two FP64 grids of scratch
+int diag_gauss(tmpg_ctx* ctx, double* rho_a, double* rho_b, int npasses, double out[4]);
 int denoise_apply(tmpg_ctx* ctx, tmpg_model* m);
 }  // namespace tmpg

@@ -135,6 +135,11 @@ module m_tmpg
 			type(c_ptr), value :: ctx
 			real(c_double), intent(out) :: out(8)
 		end function
+		integer(c_int) function tmpg_gauss_residual(ctx, out) bind(C, name="tmpg_gauss_residual")
+			import :: c_int, c_ptr, c_double
+			type(c_ptr), value :: ctx
+			real(c_double), intent(out) :: out(4)
+		end function
 		integer(c_int) function tmpg_load_uniform(ctx, species, ppc, drift_gb, theta, seed) &
 				bind(C, name="tmpg_load_uniform")
 			import :: c_int, c_ptr, c_float, c_int64_t

@@ -22,7 +22,7 @@ program tristan_gpu
 	type(c_ptr) :: ctx
 	character(len=512) :: infile, outfile, arg
 	integer :: lap, k, nargs, u, lapst
-	real(c_double) :: d(8), ms(6)
+	real(c_double) :: d(8), ms(6), gs(4)
 	integer(c_int64_t) :: ni, ne
 	real(8) :: t0, t1, tacc
 	integer(8) :: c0, c1, crate
@@ -87,7 +87,9 @@ program tristan_gpu
 		endif
 		if (mod(lap, hp%interval) == 0 .or. lap == hp%last) then
 			call tmpg_check(tmpg_diag(ctx, d), "tmpg_diag", ctx)
-			write(*, '(a,i7,4(a,es14.6))') "lap ", lap, "  E ", d(1), "  B ", d(2), "  KEi ", d(3), "  KEe ", d(4)
+			call tmpg_check(tmpg_gauss_residual(ctx, gs), "tmpg_gauss_residual", ctx)
+			write(*, '(a,i7,5(a,es14.6))') "lap ", lap, "  E ", d(1), "  B ", d(2), "  KEi ", d(3), "  KEe ", d(4), &
+				"  max|divE-rho| ", gs(1)
 			if (u /= 0) write(u, '(i7,4es24.15)') lap, d(1), d(2), d(3), d(4)
 		endif
 		call maybe_write_restart(hp%rst, ctx, lap, hp%dim, hp%order, hp%nx, hp%ny, hp%nz)
