@@ -18,10 +18,16 @@ measured with HIP events around its launch on the library's stream.
 traffic: HBM bytes per launch from FETCH_SIZE + WRITE_SIZE counter runs,
 scaled by the same counters on a plain copy of known size
 (profiles/traffic_<workload>.json).
+roofline.counters: VALU busy, LDS active and bank-conflicted fractions of
+the push kernel and its VALU lane-instructions per particle, from a
+rocprofv3 --pmc pass over a short run of this script in a child process.
 cpu_baseline: the reference's own Fortran routines (oracle/_ref, compiled
-from its sources) on one core over a bounded sample of the same workload
-(64 x 32 x 32 cells, same ppc and plasma), or the C++ oracle if that binary
-is absent.
+from its sources at -O3) on the host cores, one single-rank process per core
+on its own 64 x 32 x 32-cell sample of the same plasma (core count and CPU
+model stated).
+denoised_vs_plain (default workload): step time of the 2-ppc run with the
+learned denoiser against the 32-ppc plain run (3D pair Weibel, 3rd order,
+256^3 cells), the second half of the metric.
 """
 import argparse
 import json
@@ -119,22 +125,46 @@ def make_context(w, sort_every=0):
     return ctx, ncell
 
 
-def cpu_reference_baseline(w, budget_s=20.0):
-    """reference Fortran (oracle/_ref) on a 64x32x32 sample, one core."""
+def host_cpu():
+    """(CPU model, cores this process may use, capped at the 16 host cores
+    that go with one GPU)"""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return model, max(1, min(n, 16))
+
+
+def cpu_reference_baseline(w, steps=30):
+    """The reference's own Fortran routines (oracle/_ref, built from its
+    sources at -O3, FP32) on the host cores: one single-rank process per core,
+    each on its own 64x32x32 sample of the workload's plasma (disjoint seeds),
+    all started together.  Throughput = all processes' particle-steps / the
+    slowest process's time, start-up (a 0-step run) subtracted."""
     import numpy as np
     dim, order = w["dim"], w["order"]
-    exe = os.path.join(ROOT, "oracle", "_ref", f"tristan_ref_{dim}d_dd{order}_f32")
+    exe = os.path.join(ROOT, "oracle", "_ref", f"tristan_ref_{dim}d_dd{order}_f32_o3")
+    if not os.path.exists(exe):
+        return None
+    model, ncore = host_cpu()
     sx, sy, sz = (64, 32, 32) if dim == 3 else (256, 256, 1)
     nm = normalisation(w)
     ng = 5 if order == 1 else 7
     ngz = 5 if dim == 2 else ng
     mx, my, mz = sx + ng, sy + ng, (sz + ngz if dim == 3 else 1)
     h, hz = ng // 2, ngz // 2
-    rng = np.random.default_rng(0)
     ncell = sx * sy * sz
     n = ncell * w["ppc"]
 
-    def species(sign):
+    def species(rng):
         pos = np.empty((3, n))
         pos[0] = h + 1 + rng.random(n) * sx
         pos[1] = h + 1 + rng.random(n) * sy
@@ -145,8 +175,6 @@ def cpu_reference_baseline(w, budget_s=20.0):
         mom[0, n // 2:] += w["gb"]
         return np.concatenate([pos, mom, np.ones((1, n))]).astype(np.float32)
 
-    if not os.path.exists(exe):
-        return None
     inp = f"""<node_configuration>
 sizex = 1
 sizey = 1
@@ -179,28 +207,35 @@ user_part_bcs = 0
 """
     with tempfile.TemporaryDirectory() as td:
         open(os.path.join(td, "input"), "w").write(inp)
-        st = os.path.join(td, "state.bin")
-        with open(st, "wb") as fh:
-            fh.write(np.zeros(6 * mx * my * mz, np.float32).tobytes())
-            fh.write(np.array([n, n], np.int32).tobytes())
-            for s in (1, -1):
-                fh.write(species(s).tobytes())
+        states = []
+        for r in range(ncore):
+            rng = np.random.default_rng(1000 + r)
+            st = os.path.join(td, f"state{r}.bin")
+            with open(st, "wb") as fh:
+                fh.write(np.zeros(6 * mx * my * mz, np.float32).tobytes())
+                fh.write(np.array([n, n], np.int32).tobytes())
+                for _ in (1, -1):
+                    fh.write(species(rng).tobytes())
+            states.append(st)
 
-        def run(steps):
+        def run_all(nsteps):
             t0 = time.perf_counter()
-            subprocess.run([exe, "input", st, os.path.join(td, "out.bin"), str(steps), "0"], cwd=td, check=True,
-                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+            procs = [subprocess.Popen([exe, "input", st, os.path.join(td, f"out{r}.bin"), str(nsteps), "0"], cwd=td,
+                                      stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                     for r, st in enumerate(states)]
+            for pr in procs:
+                if pr.wait(timeout=600) != 0:
+                    raise RuntimeError(f"{exe} failed")
             return time.perf_counter() - t0
 
-        t0 = run(0)
-        t1 = run(1)
-        per_step = max(t1 - t0, 1e-6)
-        steps = max(1, min(20, int(budget_s / per_step)))
-        tk = run(steps)
-        per_step = (tk - t0) / steps
-    return dict(value=2 * n / per_step, unit="particle-steps/s", cores=1, kind="reference",
-                sample=f"reference Fortran -O2 (amdflang), {sx}x{sy}x{sz} cells, {w['ppc']} ppc/species "
-                       f"({2 * n} particles), {steps} full main-loop steps, 1 thread, startup subtracted")
+        t_start = run_all(0)
+        t_run = run_all(steps)
+        per_step = max(t_run - t_start, 1e-6) / steps
+    return dict(value=ncore * 2 * n / per_step, unit="particle-steps/s", cores=ncore, kind="reference",
+                sample=f"reference Fortran (amdflang -O3, FP32), {ncore} single-rank processes started together on "
+                       f"{ncore} host cores ({model}), each on its own {sx}x{sy}x{sz}-cell sample of this plasma "
+                       f"({w['ppc']} ppc/species, {2 * n} particles), {steps} full main-loop steps; "
+                       f"start-up subtracted")
 
 
 def read_traffic(workload):
@@ -213,13 +248,100 @@ def read_traffic(workload):
     return None
 
 
+def time_steps(w, warmup, steps):
+    """ms per full step of workload w (the library's own timing events)"""
+    ctx, _ = make_context(w)
+    try:
+        ctx.step(warmup)
+        ctx.sync()
+        ctx.last_timings()
+        t0 = time.perf_counter()
+        ctx.step(steps)
+        ctx.sync()
+        t1 = time.perf_counter()
+        npart = ctx.count(0) + ctx.count(1)
+    finally:
+        ctx.close()
+    return 1e3 * (t1 - t0) / steps, npart
+
+
+def denoise_ratio(warmup=2, steps=4):
+    """The learned-denoiser part of the metric: step time of the 2-ppc run with
+    the denoiser against the 32-ppc plain run it is judged against (3D pair
+    Weibel, 3rd order, 256^3 cells).  The network's weights do not change the
+    time; the physics at parity is checked by the denoiser tests and the
+    parity study of python/train_denoiser.py."""
+    plain_ms, n_plain = time_steps(WORKLOADS["weibel3d_ord3_256cube_ppc32"], warmup, steps)
+    den_ms, n_den = time_steps(WORKLOADS["weibel3d_ord3_256cube_ppc2_denoised"], warmup, steps)
+    return {"plain": "weibel3d_ord3_256cube_ppc32", "denoised": "weibel3d_ord3_256cube_ppc2_denoised",
+            "plain_ms_per_step": plain_ms, "denoised_ms_per_step": den_ms, "ratio": den_ms / plain_ms,
+            "target_ratio": 0.125, "particles": [n_plain, n_den], "steps": steps, "warmup": warmup}
+
+
+# counters of one profiled pass over the push kernel (limits of one rocprofv3
+# pass: at most 8 SQ and 2 GRBM counters)
+PUSH_COUNTERS = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_LDS_IDX_ACTIVE", "SQ_LDS_BANK_CONFLICT",
+                 "SQ_WAIT_INST_LDS", "GRBM_GUI_ACTIVE"]
+N_XCD, N_CU, N_SIMD = 8, 256, 1024
+
+
+def push_counters(workload, kernel, npart, timeout=240):
+    """VALU and LDS utilisation of the push kernel, from a rocprofv3 --pmc
+    pass over a short run of this script (a child process; 2 steps after 2
+    warm-up steps).  VALU busy = SQ_INSTS_VALU x 4 cycles (a wave64 VALU
+    instruction on a 16-lane SIMD) / 1024 SIMDs / the launch's cycles
+    (GRBM_GUI_ACTIVE / 8 XCDs); LDS active and bank-conflicted:
+    SQ_LDS_IDX_ACTIVE and SQ_LDS_BANK_CONFLICT / 256 CUs / cycles.  Mean over
+    the profiled launches.  None when rocprofv3 is absent or fails."""
+    import csv
+    import glob
+    import shutil
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3")
+                                         else None)
+    if not prof:
+        return None
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        cmd = [prof, "--pmc", *PUSH_COUNTERS, "--output-format", "csv", "-d", td, "-o", "pc", "--",
+               sys.executable, os.path.abspath(__file__), "--workload", workload, "--steps", "2", "--warmup", "2",
+               "--no-cpu", "--no-ratio", "--no-counters"]
+        env = dict(os.environ, TMPDIR="/tmp")
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True,
+                           timeout=timeout)
+        files = glob.glob(os.path.join(td, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            print(f"push counters: no counter file (rocprofv3 rc {r.returncode}): {r.stderr[-400:]}",
+                  file=sys.stderr)
+            return None
+        per = {}
+        for f in files:
+            for row in csv.DictReader(open(f)):
+                if kernel not in row.get("Kernel_Name", ""):
+                    continue
+                d = per.setdefault(row["Dispatch_Id"], {})
+                d[row["Counter_Name"]] = d.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    launches = [d for d in per.values() if all(c in d for c in PUSH_COUNTERS)]
+    if not launches:
+        return None
+    mean = {c: sum(d[c] for d in launches) / len(launches) for c in PUSH_COUNTERS}
+    cycles = mean["GRBM_GUI_ACTIVE"] / N_XCD
+    return {"source": f"rocprofv3 --pmc {' '.join(PUSH_COUNTERS)}, {len(launches)} launches of {kernel}",
+            "valu_busy": mean["SQ_INSTS_VALU"] * 4.0 / N_SIMD / cycles,
+            "lds_active": mean["SQ_LDS_IDX_ACTIVE"] / N_CU / cycles,
+            "lds_bank_conflicted": mean["SQ_LDS_BANK_CONFLICT"] / N_CU / cycles,
+            "valu_lane_insts_per_particle": mean["SQ_INSTS_VALU"] * 64.0 / npart,
+            "lds_lane_insts_per_particle": mean["SQ_INSTS_LDS"] * 64.0 / npart,
+            "waves_per_launch": mean["SQ_WAVES"], "cycles_per_launch": cycles}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="weibel3d_ord3_512x256x256_ppc4", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-ratio", action="store_true", help="skip the denoised / plain step-time ratio")
+    ap.add_argument("--no-counters", action="store_true", help="skip the profiled pass for VALU / LDS counters")
     ap.add_argument("--sort-every", type=int, default=0,
                     help="steps between in-tile cell sorts of the particles (0: library default, <0: never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -262,9 +384,27 @@ def main():
     alg_bytes = (56.0 if w["dim"] == 3 else 48.0) * npart + 48.0 * ncell
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     traffic = read_traffic(a.workload)
+    ctx.close()                 # free the device memory before the other runs
+    print(f"bench: {a.workload} {1e3 * dt / a.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    headline = a.workload == "weibel3d_ord3_512x256x256_ppc4"
+    ratio = None
+    if headline and not a.no_ratio:
+        try:
+            ratio = denoise_ratio()
+            print(f"bench: denoised / plain step time {ratio['ratio']:.3f}", file=sys.stderr, flush=True)
+        except Exception as e:  # reported, never required
+            print(f"denoise ratio failed: {e}", file=sys.stderr)
+    counters = None
+    if not a.no_counters:
+        try:
+            counters = push_counters(a.workload, "k_push_cells" if w["dim"] == 3 else "k_push_tiled", npart)
+            print(f"bench: push counters {counters}", file=sys.stderr, flush=True)
+        except Exception as e:  # reported, never required
+            print(f"push counters failed: {e}", file=sys.stderr)
     cpu = None
     if not a.no_cpu:
         try:
+            print("bench: CPU baseline", file=sys.stderr, flush=True)
             cpu = cpu_reference_baseline(w)
         except Exception as e:  # the baseline is reported, never required
             print(f"cpu baseline failed: {e}", file=sys.stderr)
@@ -289,9 +429,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_push_cells" if w["dim"] == 3 else "k_push_tiled",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": alg_bytes},
+                     "algorithmic_bytes": alg_bytes, "counters": counters},
         "cpu_baseline": cpu,
     }
+    if ratio is not None:
+        out["denoised_vs_plain"] = ratio
     print(json.dumps(out))
 
 

@@ -1,6 +1,7 @@
 """Energy histories of the compiled reference over many main-loop steps.
 
-A counter-streaming pair-plasma (Weibel) setup is run by the reference's own
+A counter-streaming plasma (pair Weibel, or electron beams over cold ions for
+the two-stream case) is run by the reference's own
 routines (oracle/_ref/tristan_ref_*, FP32 and FP64 builds), writing every
 EVERY steps 0.5*sum(E^2), 0.5*sum(B^2) and the kinetic energy of each
 species.  The initial state is regenerated from its seed by `initial_state`
@@ -39,6 +40,16 @@ CASES = {
     # the same two in 3D, order 3, 16^3 cells, 4 ppc per species
     "weibel3d_ord3_highorder": dict(dim=3, order=3, nx=16, ny=16, nz=16, ppc=4, steps=300, seed=77, highorder=1),
     "weibel3d_ord3_vay": dict(dim=3, order=3, nx=16, ny=16, nz=16, ppc=4, steps=300, seed=78, pusher="vay"),
+    # the 3D pair Weibel precision study (512x256x256, 4 ppc per species,
+    # gamma*beta = 1, theta = 0.01, c/wp = 6 cells, 3rd order) on 16^3 cells
+    # for its full 2000 steps
+    "weibel3d_ord3_c2": dict(dim=3, order=3, nx=16, ny=16, nz=16, ppc=4, steps=2000, seed=79,
+                             gamma0=2.0 ** 0.5, gb=1.0, theta=0.01, c_omp=6.0),
+    # the 3D electron-ion two-stream (192^3, 2nd order, m_i/m_e = 16,
+    # c/wpe = 8, electron beams +-0.2c, theta_e = 1e-4, cold ions at rest,
+    # 16 ppc per species, 500 steps) on 16^3 cells
+    "twostream3d_ord2": dict(dim=3, order=2, nx=16, ny=16, nz=16, ppc=16, steps=500, seed=80, kind="twostream",
+                             gamma0=0.2, gb=0.2 / (1.0 - 0.04) ** 0.5, theta=1e-4, c_omp=8.0, mi=16.0),
 }
 PHYS = dict(c=0.45, corr=1.025, ntimes=4, c_omp=5.0, gamma0=0.5, me=1.0, mi=1.0, gb=0.5, theta=2.5e-3)
 EVERY = 10
@@ -73,10 +84,10 @@ enlarge = 0
 sigma = 0.
 maxptl0 = {maxptl}
 ppc0 = {2 * case['ppc']}
-delgam = {p['theta']}
-me = {p['me']}
-mi = {p['mi']}
-gamma0 = {p['gamma0']}
+delgam = {case.get('theta', p['theta'])}
+me = {case.get('me', p['me'])}
+mi = {case.get('mi', p['mi'])}
+gamma0 = {case.get('gamma0', p['gamma0'])}
 c_omp = {case.get('c_omp', p['c_omp'])}
 <problem>
 distr_dim = 3
@@ -91,7 +102,8 @@ def physics(case):
     """normalisation of the reference (code/particles.F90:206-235), ppc0 = 2 ppc"""
     p = dict(PHYS)
     p["ppc0"] = 2.0 * case["ppc"]
-    p["c_omp"] = case.get("c_omp", PHYS["c_omp"])
+    for k in ("c_omp", "gamma0", "gb", "theta", "me", "mi"):
+        p[k] = case.get(k, PHYS[k])
     return p
 
 
@@ -104,15 +116,19 @@ def initial_state(case):
     h, hz = ng // 2, ngz // 2
     ncell = case["nx"] * case["ny"] * case["nz"]
     n = ncell * case["ppc"]
+    p = physics(case)
     out = []
-    for _ in range(2):
+    for s in range(2):
         pos = np.empty((3, n))
         pos[0] = h + 1 + rng.random(n) * case["nx"]
         pos[1] = h + 1 + rng.random(n) * case["ny"]
         pos[2] = (hz + 1 + rng.random(n) * case["nz"]) if case["dim"] == 3 else hz + 1.5
-        mom = rng.standard_normal((3, n)) * np.sqrt(PHYS["theta"])
-        mom[0, : n // 2] -= PHYS["gb"]
-        mom[0, n // 2:] += PHYS["gb"]
+        mom = rng.standard_normal((3, n)) * np.sqrt(p["theta"])
+        if case.get("kind") == "twostream" and s == 0:
+            mom[:] = 0.0                    # cold ions at rest
+        else:
+            mom[0, : n // 2] -= p["gb"]
+            mom[0, n // 2:] += p["gb"]
         out.append(np.concatenate([pos, mom, np.ones((1, n))]).astype(np.float32))
     return cfg, out[0], out[1]
 

@@ -9,7 +9,11 @@ steps (3D, order 3; also at 12 particles per cell per species for 300
 steps, where each cell tile holds more particles than the push's deposit
 records and the multi-pass push runs; and over 600 steps with the 4th-order
 field solver, and with the Vay pusher against the reference built with
--Dvay).  Round-off differences grow in a
+-Dvay).  The 3D pair Weibel precision study (gamma*beta = 1, theta = 0.01,
+c/wp = 6, order 3) runs its full 2000 steps on 16^3 cells, and the 3D
+electron-ion two-stream (m_i/m_e = 16, electron beams +-0.2c over cold ions,
+order 2, 16 ppc per species) its 500 steps on 16^3 cells.  Round-off
+differences grow in a
 many-particle system,
 so the bar is set by the reference against itself: the library's deviation
 from the FP64 reference may be at most a small multiple of the reference
@@ -62,7 +66,7 @@ def _with_total(h):
 
 @pytest.mark.parametrize("name", ["weibel2d_ord2", "weibel3d_ord3", "weibel2d_ord1_256", "weibel3d_ord3_dense",
                                   "weibel2d_ord2_highorder", "weibel2d_ord1_vay", "weibel3d_ord3_highorder",
-                                  "weibel3d_ord3_vay"])
+                                  "weibel3d_ord3_vay", "weibel3d_ord3_c2", "twostream3d_ord2"])
 def test_energy_history_tracks_reference(name):
     import tmpg
     import make_energy_history as meh
