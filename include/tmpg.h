@@ -44,6 +44,9 @@ extern "C" {
                                (code/particles_movedeposit.F90:988-989) */
 #define TMPG_COMPAT_DEFAULT (TMPG_Q_GATHER2 | TMPG_Q_BXY_Z)
 
+#define TMPG_SCHEME_ESIRKEPOV 0
+#define TMPG_SCHEME_ZIGZAG    1
+
 typedef struct tmpg_config {
     int32_t dim;          /* 2 or 3 (reference: -DtwoD at compile time) */
     int32_t order;        /* shape order 1, 2, 3 (reference: -Ddd1/-Ddd2/-Ddd3) */
@@ -67,7 +70,13 @@ typedef struct tmpg_config {
                              advance_e_fullstep_42, code/fields.F90:1039-1361) */
     int32_t pusher;       /* 0: Boris; 1: Vay (the reference's -Dvay build,
                              code/particles_movedeposit.F90:1199-1223) */
-    int32_t reserved[2];
+    int32_t scheme;       /* 0: Esirkepov current of the shape `order` (the
+                             reference's -Ddd<n> builds); 1: the original
+                             scheme of the -Dzzag build: trilinear mover
+                             (code/particles_movedeposit.F90:98-347) and the
+                             zigzag deposit (code/particles.F90:550-669),
+                             nghost = 5; needs order = 1 */
+    int32_t reserved[1];
 } tmpg_config;
 
 typedef struct tmpg_ctx tmpg_ctx;

@@ -114,13 +114,18 @@ def mover(f6, prt, qm, c, dim):
     return prt
 
 
-def zigzag(j3, prt, q, c, dim):
+def zigzag(j3, prt, q, c, dim, start=None):
     """deposit of the moves that ended at prt (7, n) float64 into j3 (3, mz,
     my, mx): the start point is x - u c / gamma (code/particles_movedeposit.
-    F90:1385-1388) and the charge q * ch"""
+    F90:1385-1388), or the given start (3, n) -- the same point in exact
+    arithmetic, without the round-off of the recomputation -- and the charge
+    q * ch"""
     x2, y2, z2, u, v, w, ch = prt
-    ig = 1.0 / np.sqrt(1.0 + u * u + v * v + w * w)
-    x1, y1, z1 = x2 - u * ig * c, y2 - v * ig * c, z2 - w * ig * c
+    if start is None:
+        ig = 1.0 / np.sqrt(1.0 + u * u + v * v + w * w)
+        x1, y1, z1 = x2 - u * ig * c, y2 - v * ig * c, z2 - w * ig * c
+    else:
+        x1, y1, z1 = start
     qq = q * ch
     i1, i2 = np.trunc(x1).astype(np.int64), np.trunc(x2).astype(np.int64)
     j1, j2 = np.trunc(y1).astype(np.int64), np.trunc(y2).astype(np.int64)

@@ -50,6 +50,10 @@ CASES = {
     # 16 ppc per species, 500 steps) on 16^3 cells
     "twostream3d_ord2": dict(dim=3, order=2, nx=16, ny=16, nz=16, ppc=16, steps=500, seed=80, kind="twostream",
                              gamma0=0.2, gb=0.2 / (1.0 - 0.04) ** 0.5, theta=1e-4, c_omp=8.0, mi=16.0),
+    # the original scheme (-Dzzag: trilinear mover, zigzag deposit, the
+    # first-order ghost layout): 2D 64 x 48 and 3D 16^3
+    "weibel2d_zzag": dict(dim=2, order=1, nx=64, ny=48, nz=1, ppc=8, steps=1000, seed=81, scheme="zzag"),
+    "weibel3d_zzag": dict(dim=3, order=1, nx=16, ny=16, nz=16, ppc=4, steps=500, seed=82, scheme="zzag"),
 }
 PHYS = dict(c=0.45, corr=1.025, ntimes=4, c_omp=5.0, gamma0=0.5, me=1.0, mi=1.0, gb=0.5, theta=2.5e-3)
 EVERY = 10
@@ -139,6 +143,8 @@ def run_history(case, prec):
     dt = np.float32 if prec == "f32" else np.float64
     suffix = "_vay" if case.get("pusher") == "vay" else ""
     exe = os.path.join(REF_DIR, f"tristan_ref_{case['dim']}d_dd{case['order']}_{prec}{suffix}")
+    if case.get("scheme") == "zzag":
+        exe = os.path.join(REF_DIR, f"tristan_ref_{case['dim']}d_{prec}_zzag")
     maxptl = 2 * (pi.shape[1] + pe.shape[1]) + 1000
     mx, my, mz, _, _ = orc.extent(cfg)
     with tempfile.TemporaryDirectory() as td:

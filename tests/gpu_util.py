@@ -30,7 +30,7 @@ def norm64(p):
 
 
 def make_context(tmpg, dim, order, nx, ny, nz, p, ntimes=4, compat=None, track_ids=True, capacity=(0, 0),
-                 sort_every=0, highorder=0, pusher=0, tile=(0, 0, 0)):
+                 sort_every=0, highorder=0, pusher=0, tile=(0, 0, 0), scheme=0):
     nm = norm32(p)
     kw = {}
     if compat is not None:
@@ -38,7 +38,8 @@ def make_context(tmpg, dim, order, nx, ny, nz, p, ntimes=4, compat=None, track_i
     return tmpg.Context(dim, order, nx, ny, nz, ntimes=ntimes, c=p["c"], corr=p["corr"],
                         qm=(float(nm["qmi"]), float(nm["qme"])), q=(float(nm["qi"]), float(nm["qe"])),
                         mass=(float(nm["mi"]), float(nm["me"])), track_ids=track_ids, capacity=capacity,
-                        sort_every=sort_every, highorder=highorder, pusher=pusher, tile=tile, **kw)
+                        sort_every=sort_every, highorder=highorder, pusher=pusher, tile=tile, scheme=scheme,
+                        **kw)
 
 
 def sorted_by_id(prt, ids):

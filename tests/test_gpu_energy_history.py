@@ -12,7 +12,9 @@ field solver, and with the Vay pusher against the reference built with
 -Dvay).  The 3D pair Weibel precision study (gamma*beta = 1, theta = 0.01,
 c/wp = 6, order 3) runs its full 2000 steps on 16^3 cells, and the 3D
 electron-ion two-stream (m_i/m_e = 16, electron beams +-0.2c over cold ions,
-order 2, 16 ppc per species) its 500 steps on 16^3 cells.  Round-off
+order 2, 16 ppc per species) its 500 steps on 16^3 cells; the original
+scheme (trilinear mover, zigzag deposit) runs against the -Dzzag builds in
+2D (1000 steps) and 3D (500 steps).  Round-off
 differences grow in a
 many-particle system,
 so the bar is set by the reference against itself: the library's deviation
@@ -44,7 +46,8 @@ def _gpu_history(tmpg, case, p, pi, pe):
     dim, order = case["dim"], case["order"]
     ctx = make_context(tmpg, dim, order, case["nx"], case["ny"], case["nz"], p, ntimes=p["ntimes"],
                        track_ids=False, highorder=case.get("highorder", 0),
-                       pusher=1 if case.get("pusher") == "vay" else 0)
+                       pusher=1 if case.get("pusher") == "vay" else 0,
+                       scheme=1 if case.get("scheme") == "zzag" else 0)
     cfg = orc.make_cfg(dim, order, case["nx"], case["ny"], case["nz"])
     mx, my, mz, _, _ = orc.extent(cfg)
     ctx.set_fields(np.zeros((6, mz, my, mx), np.float32))
@@ -66,7 +69,8 @@ def _with_total(h):
 
 @pytest.mark.parametrize("name", ["weibel2d_ord2", "weibel3d_ord3", "weibel2d_ord1_256", "weibel3d_ord3_dense",
                                   "weibel2d_ord2_highorder", "weibel2d_ord1_vay", "weibel3d_ord3_highorder",
-                                  "weibel3d_ord3_vay", "weibel3d_ord3_c2", "twostream3d_ord2"])
+                                  "weibel3d_ord3_vay", "weibel3d_ord3_c2", "twostream3d_ord2",
+                                  "weibel2d_zzag", "weibel3d_zzag"])
 def test_energy_history_tracks_reference(name):
     import tmpg
     import make_energy_history as meh

@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(256) k_load(int32_t* cell, float* ox, float* o
     cell[p] = pack_cell(pk, i, j, k);
     ox[p] = fminf(urand(key, 0), 0.99999994f);
     oy[p] = fminf(urand(key, 1), 0.99999994f);
-    if (dim == 3) oz[p] = fminf(urand(key, 2), 0.99999994f);
+    if (oz) oz[p] = fminf(urand(key, 2), 0.99999994f);   // 3D, or the 2D zigzag scheme's z offset
     // |u'| from the tabulated Maxwell-Juttner CDF (uniform grid in CDF)
     float x = urand(key, 3) * (float)(ntab - 1);
     int b = (int)x;
@@ -477,7 +477,7 @@ int particles_load_uniform(tmpg_ctx* ctx, int s, int ppc, float drift, float the
         cp(flat.cell, old.cell, ib);
         cp(flat.ox, old.ox, fb);
         cp(flat.oy, old.oy, fb);
-        if (g.dim == 3) cp(flat.oz, old.oz, fb);
+        if (flat.oz) cp(flat.oz, old.oz, fb);
         cp(flat.u, old.u, fb);
         cp(flat.v, old.v, fb);
         cp(flat.w, old.w, fb);

@@ -66,7 +66,8 @@ __device__ __forceinline__ void load_part(const PBuf& b, int64_t s, const PackIn
     unpack_cell(pk, b.cell[s], p.i, p.j, p.k);
     p.ox = b.ox[s];
     p.oy = b.oy[s];
-    p.oz = (DIM == 3) ? b.oz[s] : 0.0f;
+    // 2D keeps a z offset only in the zigzag scheme (b.oz allocated)
+    p.oz = (DIM == 3 || b.oz != nullptr) ? b.oz[s] : 0.0f;
     p.u = b.u[s];
     p.v = b.v[s];
     p.w = b.w[s];
@@ -80,7 +81,7 @@ __device__ __forceinline__ void store_part(const PBuf& b, int64_t s, const PackI
     b.cell[s] = pack_cell(pk, p.i, p.j, p.k);
     b.ox[s] = p.ox;
     b.oy[s] = p.oy;
-    if (DIM == 3) b.oz[s] = p.oz;
+    if (DIM == 3 || b.oz != nullptr) b.oz[s] = p.oz;
     b.u[s] = p.u;
     b.v[s] = p.v;
     b.w[s] = p.w;
@@ -127,12 +128,14 @@ __device__ __forceinline__ void push_fields(const float* E, int flags, float c, 
               dz);
 }
 
+// ORD 0 is the original scheme (trilinear gather of the staggered fields)
 template <int DIM, int ORD, bool VAY_OK = true, class Src>
 __device__ __forceinline__ void push_one(const Src& src, int flags, float c, float cinv, float qm, int si, int sj,
                                          int sk, Part& p, float& dx, float& dy, float& dz)
 {
     float E[6];
-    gather<DIM, ORD>(src, (flags & PUSH_GATHER2) != 0, si, sj, sk, p.ox, p.oy, p.oz, E);
+    if constexpr (ORD == 0) gather_zz<DIM>(src, si, sj, sk, p.ox, p.oy, p.oz, E);
+    else gather<DIM, ORD>(src, (flags & PUSH_GATHER2) != 0, si, sj, sk, p.ox, p.oy, p.oz, E);
     const float se = 0.5f * qm, sb = 0.5f * qm * cinv;
     if (VAY_OK && (flags & PUSH_VAY))
         vay(c, cinv, se * E[0], se * E[1], se * E[2], sb * E[3], sb * E[4], sb * E[5], p.u, p.v, p.w, dx, dy, dz);
@@ -141,11 +144,24 @@ __device__ __forceinline__ void push_one(const Src& src, int flags, float c, flo
               dz);
 }
 
-__device__ __forceinline__ void advance(Part& p, int dim, int nx, int ny, int nz, float dx, float dy, float dz)
+__device__ __forceinline__ void advance(Part& p, int dim, int nx, int ny, int nz, float dx, float dy, float dz,
+                                        bool z_offset_2d = false)
 {
     p.i = wrap1(p.i + advance_offset(p.ox, dx), nx);
     p.j = wrap1(p.j + advance_offset(p.oy, dy), ny);
     if (dim == 3) p.k = wrap1(p.k + advance_offset(p.oz, dz), nz);
+    // the 2D zigzag scheme tracks z modulo one cell, like the reference's
+    // wrap of z into its single plane (code/particles_movedeposit.F90:1553-1633)
+    else if (z_offset_2d) (void)advance_offset(p.oz, dz);
+}
+
+// Esirkepov deposit of the shape ORD, or the zigzag deposit for ORD 0
+template <int DIM, int ORD, class Sink, bool SKIP_ZERO = true>
+__device__ __forceinline__ void deposit_any(const Sink& snk, int i, int j, int k, float ox, float oy, float oz,
+                                            float dx, float dy, float dz, float q)
+{
+    if constexpr (ORD == 0) deposit_zz<DIM>(snk, i, j, k, ox, oy, oz, dx, dy, dz, q);
+    else deposit<DIM, ORD, Sink, SKIP_ZERO>(snk, i, j, k, ox, oy, oz, dx, dy, dz, q);
 }
 
 // --------------------------------------------------------------- tiled
@@ -166,7 +182,9 @@ __global__ void __launch_bounds__(512) k_push_tiled(TiledArgs a)
     float* sf = sm;
     double* sj = reinterpret_cast<double*>(sm + ((6 * fplane + 1) & ~1));
     const int tid = threadIdx.x, NT = blockDim.x;
-    // stage E,B (primal in 3D) with the halo, wrapping periodically
+    // stage E,B (primal in 3D; the staggered fields of the zigzag scheme,
+    // ORD 0) with the halo, wrapping periodically
+    constexpr bool NODE_MAJOR = DIM == 3 && ORD != 0;
     for (int e = tid; e < fplane; e += NT) {
         const int li = e % a.FX;
         const int r = e / a.FX;
@@ -176,7 +194,7 @@ __global__ void __launch_bounds__(512) k_push_tiled(TiledArgs a)
         const int gj = wrap1(j0 - a.fo + lj, a.ny);
         const int gk = (DIM == 3) ? wrap1(k0 - a.fo + lk, a.nz) : 0;
         const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
-        if (DIM == 3) {
+        if (NODE_MAJOR) {
             // node-major: six components per node
             float2* d = reinterpret_cast<float2*>(sf) + 3 * e;
             d[0] = make_float2(a.f6[0][g], a.f6[1][g]);
@@ -190,9 +208,9 @@ __global__ void __launch_bounds__(512) k_push_tiled(TiledArgs a)
     for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
     __syncthreads();
 
-    using Src = typename std::conditional<DIM == 3, LdsSrc6, LdsSrc>::type;
+    using Src = typename std::conditional<NODE_MAJOR, LdsSrc6, LdsSrc>::type;
     Src src;
-    if constexpr (DIM == 3) src = LdsSrc6{sf, a.FX, a.FY};
+    if constexpr (NODE_MAJOR) src = LdsSrc6{sf, a.FX, a.FY};
     else src = LdsSrc{sf, a.FX, a.FY, fplane};
     const LdsSink snk{sj, a.JX, a.JY, jplane};
     const int lane = __lane_id();
@@ -216,9 +234,9 @@ __global__ void __launch_bounds__(512) k_push_tiled(TiledArgs a)
                 float dx, dy, dz;
                 push_one<DIM, ORD>(src, a.flags, a.c, a.cinv, sp.qm, li + a.fo, lj + a.fo,
                                    (DIM == 3) ? lk + a.fo : 0, p, dx, dy, dz);
-                advance(p, DIM, a.nx, a.ny, a.nz, dx, dy, dz);
-                deposit<DIM, ORD>(snk, li + 2, lj + 2, (DIM == 3) ? lk + 2 : 0, ox, oy, oz, dx, dy, dz,
-                                  sp.q * p.ch);
+                advance(p, DIM, a.nx, a.ny, a.nz, dx, dy, dz, ORD == 0);
+                deposit_any<DIM, ORD>(snk, li + 2, lj + 2, (DIM == 3) ? lk + 2 : 0, ox, oy, oz, dx, dy, dz,
+                                      sp.q * p.ch);
                 (void)oi; (void)oj; (void)ok;
                 td = tile_of(tl, p.i, p.j, p.k);
             }
@@ -846,7 +864,7 @@ __global__ void __launch_bounds__(256) k_push_flat(FlatArgs a)
     if (MODE == 2) {
         int i, j, k;
         unpack_cell(a.pk, a.cell_old[sc], i, j, k);
-        deposit<DIM, ORD>(a.snk, i, j, k, a.ox_old[sc], a.oy_old[sc], DIM == 3 ? a.oz_old[sc] : 0.0f, a.dxs[sc],
+        deposit_any<DIM, ORD>(a.snk, i, j, k, a.ox_old[sc], a.oy_old[sc], a.oz_old ? a.oz_old[sc] : 0.0f, a.dxs[sc],
                           a.dys[sc], a.dzs[sc], a.sp.q * p.ch);
         return;
     }
@@ -854,8 +872,8 @@ __global__ void __launch_bounds__(256) k_push_flat(FlatArgs a)
     const float ox = p.ox, oy = p.oy, oz = p.oz;
     float dx, dy, dz;
     push_one<DIM, ORD>(a.src, a.flags, a.c, a.cinv, a.sp.qm, p.i, p.j, p.k, p, dx, dy, dz);
-    advance(p, DIM, a.nx, a.ny, a.nz, dx, dy, dz);
-    if (MODE == 0) deposit<DIM, ORD>(a.snk, oi, oj, ok, ox, oy, oz, dx, dy, dz, a.sp.q * p.ch);
+    advance(p, DIM, a.nx, a.ny, a.nz, dx, dy, dz, ORD == 0);
+    if (MODE == 0) deposit_any<DIM, ORD>(a.snk, oi, oj, ok, ox, oy, oz, dx, dy, dz, a.sp.q * p.ch);
     const int64_t td = tile_of(a.tl, p.i, p.j, p.k);
     const int32_t slot = atomicAdd(a.sp.cnt_b + td, 1);
     const int64_t slots = (int64_t)a.tl.ntiles * a.sp.tcap;
@@ -864,7 +882,7 @@ __global__ void __launch_bounds__(256) k_push_flat(FlatArgs a)
         a.cell_old[d] = pack_cell(a.pk, oi, oj, ok);
         a.ox_old[d] = ox;
         a.oy_old[d] = oy;
-        if (DIM == 3) a.oz_old[d] = oz;
+        if (a.oz_old) a.oz_old[d] = oz;
         a.dxs[d] = dx;
         a.dys[d] = dy;
         a.dzs[d] = dz;
@@ -1072,15 +1090,21 @@ void dispatch(int dim, int order, bool ch, bool id, F&& f)
     };
     using I2 = std::integral_constant<int, 2>;
     using I3 = std::integral_constant<int, 3>;
+    using O0 = std::integral_constant<int, 0>;   // the zigzag scheme
     using O1 = std::integral_constant<int, 1>;
     using O2 = std::integral_constant<int, 2>;
     using O3 = std::integral_constant<int, 3>;
     if (dim == 2) {
-        if (order == 1) go(I2{}, O1{}); else if (order == 2) go(I2{}, O2{}); else go(I2{}, O3{});
+        if (order == 0) go(I2{}, O0{});
+        else if (order == 1) go(I2{}, O1{}); else if (order == 2) go(I2{}, O2{}); else go(I2{}, O3{});
     } else {
-        if (order == 1) go(I3{}, O1{}); else if (order == 2) go(I3{}, O2{}); else go(I3{}, O3{});
+        if (order == 0) go(I3{}, O0{});
+        else if (order == 1) go(I3{}, O1{}); else if (order == 2) go(I3{}, O2{}); else go(I3{}, O3{});
     }
 }
+
+// template order of the push kernels: the shape order, or 0 for the zigzag scheme
+int push_order(const tmpg_ctx* ctx) { return ctx->zz ? 0 : ctx->cfg.order; }
 
 int push_flags(const tmpg_ctx* ctx)
 {
@@ -1114,7 +1138,7 @@ FlatArgs flat_args(tmpg_ctx* ctx, int s)
     a.c = ctx->cfg.c;
     a.cinv = 1.0f / ctx->cfg.c;
     a.flags = push_flags(ctx);
-    for (int q = 0; q < 6; ++q) a.src.f[q] = (g.dim == 3) ? ctx->prim[q] : ctx->f[EX + q];
+    for (int q = 0; q < 6; ++q) a.src.f[q] = (g.dim == 3 && !ctx->zz) ? ctx->prim[q] : ctx->f[EX + q];
     a.src.nx = g.nx; a.src.ny = g.ny; a.src.nz = g.nz;
     for (int q = 0; q < 3; ++q) a.snk.j[q] = ctx->f[JX + q];
     a.snk.nx = g.nx; a.snk.ny = g.ny; a.snk.nz = g.nz;
@@ -1164,7 +1188,7 @@ int alloc_pbuf(tmpg_ctx* ctx, PBuf& b, int64_t n, bool has_ch, bool has_id)
     rc |= al((void**)&b.cell, ib);
     rc |= al((void**)&b.ox, fb);
     rc |= al((void**)&b.oy, fb);
-    if (ctx->g.dim == 3) rc |= al((void**)&b.oz, fb);
+    if (ctx->g.dim == 3 || ctx->zz) rc |= al((void**)&b.oz, fb);   // 2D zigzag: z modulo one cell
     rc |= al((void**)&b.u, fb);
     rc |= al((void**)&b.v, fb);
     rc |= al((void**)&b.w, fb);
@@ -1311,6 +1335,9 @@ int particles_sort_tiles(tmpg_ctx* ctx)
 {
     const Grid& g = ctx->g;
     const Tiling& tl = ctx->tl;
+    // the sort carries the 3D offsets; the 2D zigzag scheme's z offset is
+    // not among its arrays, so its particles keep their order
+    if (g.dim == 2 && ctx->zz) return TMPG_OK;
     for (int s = 0; s < 2; ++s) {
         Species& sp = ctx->sp[s];
         if (!sp.cnt_a || sp.n == 0) continue;
@@ -1367,7 +1394,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
 {
     const Grid& g = ctx->g;
     const Tiling& tl = ctx->tl;
-    if (g.dim == 3 && mode != 2) {
+    if (g.dim == 3 && mode != 2 && !ctx->zz) {
         int rc = fields_primal(ctx);
         if (rc) return rc;
     }
@@ -1388,10 +1415,13 @@ int particles_push(tmpg_ctx* ctx, int mode)
         a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
         a.pk = ctx->pk;
         a.tl = tl;
-        a.fo = (g.dim == 3) ? 1 : 2;
-        a.FX = tl.tx + (g.dim == 3 ? 3 : 4);
-        a.FY = tl.ty + (g.dim == 3 ? 3 : 4);
-        a.FZ = (g.dim == 3) ? tl.tz + 3 : 1;
+        // staged fields: 3D primal nodes -1 .. T+1; 2D and the zigzag
+        // scheme's staggered fields -2 .. T+1
+        const bool prim = g.dim == 3 && !ctx->zz;
+        a.fo = prim ? 1 : 2;
+        a.FX = tl.tx + (prim ? 3 : 4);
+        a.FY = tl.ty + (prim ? 3 : 4);
+        a.FZ = (g.dim == 3) ? tl.tz + (prim ? 3 : 4) : 1;
         a.JX = tl.tx + 5;
         a.JY = tl.ty + 5;
         a.JZ = (g.dim == 3) ? tl.tz + 5 : 1;
@@ -1399,7 +1429,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
         a.cinv = 1.0f / ctx->cfg.c;
         a.flags = push_flags(ctx);
         a.remap = (tl.ntiles % 8 == 0) ? 1 : 0;
-        for (int q = 0; q < 6; ++q) a.f6[q] = (g.dim == 3) ? ctx->prim[q] : ctx->f[EX + q];
+        for (int q = 0; q < 6; ++q) a.f6[q] = prim ? ctx->prim[q] : ctx->f[EX + q];
         for (int q = 0; q < 3; ++q) a.j3[q] = ctx->f[JX + q];
         for (int s = 0; s < 2; ++s) a.sp[s] = spec_args(ctx, s);
         // a species without particles still needs valid count arrays
@@ -1418,7 +1448,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
         // the cell-sum kernel is compiled for the default 8x8x4 tile; other
         // tilings (explicit, or clamped on grids under 8x8x4 cells) take the
         // general tiled kernel
-        const bool cells = g.dim == 3 && ctx->use_cell_sums && tl.tx == 8 && tl.ty == 8 && tl.tz == 4 &&
+        const bool cells = prim && ctx->use_cell_sums && tl.tx == 8 && tl.ty == 8 && tl.tz == 4 &&
                            CL.bytes <= 160 * 1024;
         (void)ncell_tile;
         const bool vay = ctx->cfg.pusher == 1;
@@ -1431,10 +1461,14 @@ int particles_push(tmpg_ctx* ctx, int mode)
             TMPG_HIP_CHECK(ctx, hipEventRecord(ctx->push_events[0], ctx->stream));
             ctx->push_recorded = true;
         }
-        dispatch(g.dim, ctx->cfg.order, ch, id, [&](auto D, auto O, auto C, auto I) {
+        dispatch(g.dim, push_order(ctx), ch, id, [&](auto D, auto O, auto C, auto I) {
             constexpr int Dv = decltype(D)::value, Ov = decltype(O)::value;
             constexpr bool Cv = decltype(C)::value, Iv = decltype(I)::value;
-            if (Dv == 3 && cells) {
+            if constexpr (Ov == 0) {
+                auto k = k_push_tiled<Dv, Ov, Cv, Iv>;
+                (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                hipLaunchKernelGGL(k, dim3((unsigned)tl.ntiles), dim3(512), lds, ctx->stream, a);
+            } else if (Dv == 3 && cells) {
                 // tiles that hold more particles than deposit records on
                 // average get the kernel that drains its records in passes
                 // Vay runs get their own instantiations, so that the Boris
@@ -1464,7 +1498,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
             f.lcount = sp.ocnt;
             f.lcap = sp.ocap;
             f.nthreads = sp.ocap;
-            dispatch(g.dim, ctx->cfg.order, sp.has_ch, sp.has_id, [&](auto D, auto O, auto C, auto I) {
+            dispatch(g.dim, push_order(ctx), sp.has_ch, sp.has_id, [&](auto D, auto O, auto C, auto I) {
                 launch_flat_mode<decltype(D)::value, decltype(O)::value, decltype(C)::value, decltype(I)::value>(
                     f, 0, ctx->stream);
             });
@@ -1483,7 +1517,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
                     f.in = sp.oa; f.lcount = sp.ocnt; f.lcap = sp.ocap; f.nthreads = sp.ocap;
                     f.scratch_off = sp.slots;
                 }
-                dispatch(g.dim, ctx->cfg.order, sp.has_ch, sp.has_id, [&](auto D, auto O, auto C, auto I) {
+                dispatch(g.dim, push_order(ctx), sp.has_ch, sp.has_id, [&](auto D, auto O, auto C, auto I) {
                     launch_flat_mode<decltype(D)::value, decltype(O)::value, decltype(C)::value,
                                      decltype(I)::value>(f, mode, ctx->stream);
                 });

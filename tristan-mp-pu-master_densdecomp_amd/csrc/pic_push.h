@@ -303,6 +303,108 @@ __device__ __forceinline__ void deposit(const Sink& snk, int i, int j, int k, fl
     }
 }
 
+// ------------------------------------------------- the original scheme
+// The -Dzzag build of the reference: its mover interpolates the staggered
+// fields trilinearly with the half-cell staggering folded in
+// (code/particles_movedeposit.F90:98-347) and its deposit is the zigzag
+// scheme of Umeda et al. 2003 (code/particles.F90:550-669).
+
+// E, B at the particle for the zigzag mover: the staggered component along an
+// axis is the two half-cell neighbours' sum plus the offset times their
+// difference across two cells, then bilinear in the other axes (E) or
+// summed once more over the second staggered axis (B).  In 2D every z offset
+// is the particle's own plane.  The sums carry a factor 2 (E) and 4 (B),
+// removed here, so E[] is on the scale of gather()'s output.
+template <int DIM, class Src>
+__device__ __forceinline__ void gather_zz(const Src& src, int i, int j, int k, float dx, float dy, float dz,
+                                          float* E)
+{
+    auto at = [&](int q, int a, int b, int c) {
+        return src.get(q, src.row(j + b, DIM == 3 ? k + c : 0), src.col(i + a));
+    };
+    auto sx = [&](int q, int b, int c) {
+        return at(q, 0, b, c) + at(q, -1, b, c) + dx * (at(q, 1, b, c) - at(q, -1, b, c));
+    };
+    auto sy = [&](int q, int a, int c) {
+        return at(q, a, 0, c) + at(q, a, -1, c) + dy * (at(q, a, 1, c) - at(q, a, -1, c));
+    };
+    auto sz = [&](int q, int a, int b) {
+        return at(q, a, b, 0) + at(q, a, b, -1) + dz * (at(q, a, b, 1) - at(q, a, b, -1));
+    };
+    auto lerp = [](float p, float r, float t) { return p + t * (r - p); };
+    const float ex = lerp(lerp(sx(0, 0, 0), sx(0, 1, 0), dy), lerp(sx(0, 0, 1), sx(0, 1, 1), dy), dz);
+    const float ey = lerp(lerp(sy(1, 0, 0), sy(1, 0, 1), dz), lerp(sy(1, 1, 0), sy(1, 1, 1), dz), dx);
+    const float ez = lerp(lerp(sz(2, 0, 0), sz(2, 1, 0), dx), lerp(sz(2, 0, 1), sz(2, 1, 1), dx), dy);
+    // Bx: staggered in y and z; By: in z and x; Bz: in x and y
+    float f = sz(3, 0, 0) + sz(3, 0, -1) + dy * (sz(3, 0, 1) - sz(3, 0, -1));
+    float g = sz(3, 1, 0) + sz(3, 1, -1) + dy * (sz(3, 1, 1) - sz(3, 1, -1));
+    const float bx = lerp(f, g, dx);
+    f = sx(4, 0, 0) + sx(4, 0, -1) + dz * (sx(4, 0, 1) - sx(4, 0, -1));
+    g = sx(4, 1, 0) + sx(4, 1, -1) + dz * (sx(4, 1, 1) - sx(4, 1, -1));
+    const float by = lerp(f, g, dy);
+    f = sy(5, 0, 0) + sy(5, -1, 0) + dx * (sy(5, 1, 0) - sy(5, -1, 0));
+    g = sy(5, 0, 1) + sy(5, -1, 1) + dx * (sy(5, 1, 1) - sy(5, -1, 1));
+    const float bz = lerp(f, g, dz);
+    E[0] = 0.5f * ex;
+    E[1] = 0.5f * ey;
+    E[2] = 0.5f * ez;
+    E[3] = 0.25f * bx;
+    E[4] = 0.25f * by;
+    E[5] = 0.25f * bz;
+}
+
+// relay point of one axis (positions relative to the old cell: a1 in [0,1),
+// a2 = a1 + displacement, in cell n2 = floor(a2)): the midpoint, clamped to
+// the cell boundary the move crosses
+__device__ __forceinline__ float zz_relay(float a1, float a2, int n2)
+{
+    const float lo = (float)((n2 < 0 ? n2 : 0) + 1), hi = (float)(n2 > 0 ? n2 : 0);
+    return fminf(lo, fmaxf(hi, 0.5f * (a1 + a2)));
+}
+
+// zigzag deposit of one particle from old cell (i,j,k) / offsets o*, with
+// displacement d*, charge q: each half of the move (start -> relay, relay ->
+// end) deposits its flux with area weights at its centre.  Same stored sign
+// as deposit() (E += cur).  In 2D, z only splits the z flux at the relay
+// point (the reference tracks z modulo one cell there).
+template <int DIM, class Sink>
+__device__ __forceinline__ void deposit_zz(const Sink& snk, int i, int j, int k, float ox, float oy, float oz,
+                                           float dx, float dy, float dz, float q)
+{
+    const float x2 = ox + dx, y2 = oy + dy, z2 = oz + dz;
+    const int i2 = (int)floorf(x2), j2 = (int)floorf(y2), k2 = (int)floorf(z2);
+    const float xr = zz_relay(ox, x2, i2), yr = zz_relay(oy, y2, j2), zr = zz_relay(oz, z2, k2);
+    const float fx1 = -q * (xr - ox), fy1 = -q * (yr - oy), fz1 = -q * (zr - oz);
+    const float fx2 = -q * (x2 - xr), fy2 = -q * (y2 - yr), fz2 = -q * (z2 - zr);
+    const float wx1 = 0.5f * (ox + xr), wy1 = 0.5f * (oy + yr);
+    const float wx2 = 0.5f * (x2 + xr) - (float)i2, wy2 = 0.5f * (y2 + yr) - (float)j2;
+    const float wz1 = DIM == 3 ? 0.5f * (oz + zr) : 0.0f;
+    const float wz2 = DIM == 3 ? 0.5f * (z2 + zr) - (float)k2 : 0.0f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int ci = i + (h ? i2 : 0), cj = j + (h ? j2 : 0), ck = DIM == 3 ? k + (h ? k2 : 0) : 0;
+        const float fx = h ? fx2 : fx1, fy = h ? fy2 : fy1, fz = h ? fz2 : fz1;
+        const float wx = h ? wx2 : wx1, wy = h ? wy2 : wy1, wz = h ? wz2 : wz1;
+        const auto r00 = snk.row(cj, ck), r10 = snk.row(cj + 1, ck);
+        const int c0 = snk.col(ci), c1 = snk.col(ci + 1);
+        snk.add(0, r00, c0, fx * (1.0f - wy) * (1.0f - wz));
+        snk.add(0, r10, c0, fx * wy * (1.0f - wz));
+        snk.add(1, r00, c0, fy * (1.0f - wx) * (1.0f - wz));
+        snk.add(1, r00, c1, fy * wx * (1.0f - wz));
+        if (DIM == 3) {
+            const auto r01 = snk.row(cj, ck + 1), r11 = snk.row(cj + 1, ck + 1);
+            snk.add(0, r01, c0, fx * (1.0f - wy) * wz);
+            snk.add(0, r11, c0, fx * wy * wz);
+            snk.add(1, r01, c0, fy * (1.0f - wx) * wz);
+            snk.add(1, r01, c1, fy * wx * wz);
+        }
+        snk.add(2, r00, c0, fz * (1.0f - wx) * (1.0f - wy));
+        snk.add(2, r00, c1, fz * wx * (1.0f - wy));
+        snk.add(2, r10, c0, fz * (1.0f - wx) * wy);
+        snk.add(2, r10, c1, fz * wx * wy);
+    }
+}
+
 // One z node plane (window slot zc, 0 <= zc < W) of the 3D Esirkepov deposit
 // of one particle: exactly the values deposit<3, ORD, Sink, false> adds in
 // that plane.  Splitting a particle's deposit over its W planes lets W

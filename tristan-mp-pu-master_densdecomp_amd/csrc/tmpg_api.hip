@@ -57,7 +57,7 @@ int alloc_split_scratch(tmpg_ctx* ctx, Species& s)
     rc |= dalloc(ctx, &s.cell_old, cap);
     rc |= dalloc(ctx, &s.ox_old, cap);
     rc |= dalloc(ctx, &s.oy_old, cap);
-    if (ctx->g.dim == 3) rc |= dalloc(ctx, &s.oz_old, cap);
+    if (ctx->g.dim == 3 || ctx->zz) rc |= dalloc(ctx, &s.oz_old, cap);
     rc |= dalloc(ctx, &s.dx, cap);
     rc |= dalloc(ctx, &s.dy, cap);
     rc |= dalloc(ctx, &s.dz, cap);
@@ -138,6 +138,15 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
     if (cfg->dim != 2 && cfg->dim != 3) { g_create_err = "dim must be 2 or 3"; return TMPG_EINVAL; }
     if (cfg->order < 1 || cfg->order > 3) { g_create_err = "order must be 1, 2 or 3"; return TMPG_EINVAL; }
     if (cfg->pusher < 0 || cfg->pusher > 1) { g_create_err = "pusher must be 0 (Boris) or 1 (Vay)"; return TMPG_EINVAL; }
+    if (cfg->scheme != TMPG_SCHEME_ESIRKEPOV && cfg->scheme != TMPG_SCHEME_ZIGZAG) {
+        g_create_err = "scheme must be 0 (Esirkepov) or 1 (zigzag)";
+        return TMPG_EINVAL;
+    }
+    if (cfg->scheme == TMPG_SCHEME_ZIGZAG && (cfg->order != 1 || cfg->pusher != 0)) {
+        // the -Dzzag build: first-order ghost layout, the Boris push of its mover
+        g_create_err = "the zigzag scheme needs order = 1 and the Boris pusher";
+        return TMPG_EINVAL;
+    }
     const int nz = cfg->dim == 3 ? cfg->nz : 1;
     if (cfg->nx < 4 || cfg->ny < 4 || nz < 1 || (cfg->dim == 3 && nz < 4)) {
         g_create_err = "grid too small (need at least 4 cells per periodic axis)";
@@ -150,6 +159,7 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
     if (!ctx) { g_create_err = "out of host memory"; return TMPG_ENOMEM; }
     ctx->cfg = *cfg;
     ctx->cfg.nz = nz;
+    ctx->zz = cfg->scheme == TMPG_SCHEME_ZIGZAG;
     ctx->sort_every = cfg->sort_every == 0 ? 4 : cfg->sort_every;
     ctx->g.dim = cfg->dim;
     ctx->g.nx = cfg->nx;
@@ -188,7 +198,7 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
         tx = std::min(tx, cfg->nx);
         ty = std::min(ty, cfg->ny);
         tz = std::min(tz, nz);
-        const size_t lds = sizeof(float) * (6 * (size_t)(tx + 4) * (ty + 4) * (cfg->dim == 3 ? tz + 3 : 1) + 1) +
+        const size_t lds = sizeof(float) * (6 * (size_t)(tx + 4) * (ty + 4) * (cfg->dim == 3 ? tz + 4 : 1) + 1) +
                            sizeof(double) * 3 * (size_t)(tx + 5) * (ty + 5) * (cfg->dim == 3 ? tz + 5 : 1);
         if (lds > 160 * 1024) {
             g_create_err = "tile too large for the 160 KiB LDS of a workgroup";
@@ -330,7 +340,9 @@ int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, co
     const Grid& g = ctx->g;
     const double base_xy = ctx->h + 1, base_z = ctx->hz + 1;
     for (int64_t p = 0; p < n; ++p) {
-        double r[3] = {(double)x[p] - base_xy, (double)y[p] - base_xy, g.dim == 3 ? (double)z[p] - base_z : 0.0};
+        // 2D keeps z only in the zigzag scheme, as an offset in its one cell
+        const bool zin = g.dim == 3 || (ctx->zz && z != nullptr);
+        double r[3] = {(double)x[p] - base_xy, (double)y[p] - base_xy, zin ? (double)z[p] - base_z : 0.5};
         const int nn[3] = {g.nx, g.ny, g.nz};
         int c[3];
         for (int a = 0; a < 3; ++a) {
@@ -360,7 +372,7 @@ int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, co
     rc |= up(flat.cell, cell.data(), (size_t)n * sizeof(int32_t));
     rc |= up(flat.ox, o[0].data(), fb);
     rc |= up(flat.oy, o[1].data(), fb);
-    if (g.dim == 3) rc |= up(flat.oz, o[2].data(), fb);
+    if (flat.oz) rc |= up(flat.oz, o[2].data(), fb);
     rc |= up(flat.u, u, fb);
     rc |= up(flat.v, v, fb);
     rc |= up(flat.w, w, fb);
@@ -426,7 +438,7 @@ int tmpg_get_particles(tmpg_ctx* ctx, int species, int64_t n_capacity, int64_t* 
         rc |= down(cell.data(), flat.cell, (size_t)m * sizeof(int32_t));
         rc |= down(o[0].data(), flat.ox, fb);
         rc |= down(o[1].data(), flat.oy, fb);
-        if (g.dim == 3) rc |= down(o[2].data(), flat.oz, fb);
+        if (flat.oz) rc |= down(o[2].data(), flat.oz, fb);
         if (u) rc |= down(u, flat.u, fb);
         if (v) rc |= down(v, flat.v, fb);
         if (w) rc |= down(w, flat.w, fb);
@@ -444,7 +456,8 @@ int tmpg_get_particles(tmpg_ctx* ctx, int species, int64_t n_capacity, int64_t* 
         unpack_cell(ctx->pk, cell[(size_t)p], i, j, k);
         if (x) x[p] = (float)(bxy + i + (double)o[0][(size_t)p]);
         if (y) y[p] = (float)(bxy + j + (double)o[1][(size_t)p]);
-        if (z) z[p] = (float)(g.dim == 3 ? bz + k + (double)o[2][(size_t)p] : bz + 0.5);
+        if (z) z[p] = (float)(g.dim == 3 ? bz + k + (double)o[2][(size_t)p]
+                                         : bz + (ctx->zz ? (double)o[2][(size_t)p] : 0.5));
     }
     return TMPG_OK;
 }

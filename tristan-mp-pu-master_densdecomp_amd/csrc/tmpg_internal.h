@@ -98,6 +98,7 @@ struct tmpg_ctx {
     tmpg::Species sp[2];
     bool move_pending = false;
     bool use_cell_sums = true;  // 3D: per-cell register sums of the deposit
+    bool zz = false;            // the original scheme (cfg.scheme == TMPG_SCHEME_ZIGZAG)
     tmpg_model* denoiser = nullptr;  // replaces the filter passes when set
     // diagnostics (FP64)
     double* rho = nullptr;

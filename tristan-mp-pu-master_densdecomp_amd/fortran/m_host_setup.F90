@@ -127,6 +127,7 @@ contains
 		cfg%sort_every = 0
 		cfg%highorder = merge(1, 0, hp%highorder /= 0)
 		cfg%pusher = hp%pusher
+		cfg%scheme = 0
 		cfg%reserved = 0
 	end subroutine make_config
 

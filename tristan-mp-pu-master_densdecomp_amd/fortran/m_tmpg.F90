@@ -22,7 +22,8 @@ module m_tmpg
 		integer(c_int32_t) :: sort_every
 		integer(c_int32_t) :: highorder
 		integer(c_int32_t) :: pusher
-		integer(c_int32_t) :: reserved(2)
+		integer(c_int32_t) :: scheme
+		integer(c_int32_t) :: reserved(1)
 	end type tmpg_config
 
 	interface

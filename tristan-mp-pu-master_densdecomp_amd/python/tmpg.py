@@ -61,7 +61,8 @@ class Config(ctypes.Structure):
         ("sort_every", ctypes.c_int32),
         ("highorder", ctypes.c_int32),
         ("pusher", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("scheme", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -102,7 +103,7 @@ class Context:
 
     def __init__(self, dim, order, nx, ny, nz=1, ntimes=4, compat=COMPAT_DEFAULT, c=0.45, corr=1.025,
                  qm=(1.0, -1.0), q=(1.0, -1.0), mass=(1.0, 1.0), capacity=(0, 0), track_ids=False,
-                 tile=(0, 0, 0), sort_every=0, highorder=0, pusher=0):
+                 tile=(0, 0, 0), sort_every=0, highorder=0, pusher=0, scheme=0):
         self.lib = load_library()
         cfg = Config()
         cfg.dim, cfg.order, cfg.nx, cfg.ny, cfg.nz = dim, order, nx, ny, (nz if dim == 3 else 1)
@@ -115,6 +116,7 @@ class Context:
         cfg.sort_every = sort_every
         cfg.highorder = int(bool(highorder))
         cfg.pusher = int(pusher)
+        cfg.scheme = int(scheme)
         self.cfg = cfg
         self.dim, self.order = dim, order
         self.nx, self.ny, self.nz = nx, ny, (nz if dim == 3 else 1)
