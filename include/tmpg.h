@@ -163,6 +163,15 @@ int tmpg_diag(tmpg_ctx* ctx, double out[8]);
  * keeps its initial value up to round-off; a neutral start (E = 0, the
  * species at the same places) keeps it near zero. */
 int tmpg_gauss_residual(tmpg_ctx* ctx, double out[4]);
+/* tmpg_gauss_residual without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * kernels work in the library's static per-GPU scratch (2^22 cells of two
+ * FP64 grids, 64 MiB, held by each GPU the library runs on until the process
+ * ends) and clear it with kernels, not memsets.  Grids of more than 2^22
+ * cells: TMPG_EINVAL.  Calls on one GPU share that scratch: calls that
+ * overlap in time on one GPU (several host threads, several streams) race. */
+int tmpg_gauss_residual_noalloc(tmpg_ctx* ctx, double out[4]);
 /* charge density at the nodes (interior nx*ny*nz, x fastest), FP64,
  * deposited with the order's own shape from the current positions */
 int tmpg_charge_density(tmpg_ctx* ctx, double* rho);

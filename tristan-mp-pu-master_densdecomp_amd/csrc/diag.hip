@@ -8,8 +8,10 @@
 // (code/particles.F90:2549-2938): uniform positions per cell, a tabulated
 // Maxwell-Juttner speed distribution (init_maxw_table) and Zenitani's
 // flipping method for the drift.
-#include <vector>
 #include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "pic_math.h"
 #include "tmpg_internal.h"
@@ -351,10 +353,31 @@ int diag_energies(tmpg_ctx* ctx, double out[4])
     return TMPG_OK;
 }
 
-int diag_rho(tmpg_ctx* ctx, double* rho)
+// zero n doubles with a kernel (no memset node, so a captured sequence
+// replays it like any other kernel)
+__global__ void __launch_bounds__(256) k_zero_f64(double* __restrict__ a, int64_t n)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        a[t] = 0.0;
+}
+
+static int zero_f64(tmpg_ctx* ctx, double* a, int64_t n, bool by_kernel)
+{
+    if (!by_kernel) {
+        TMPG_HIP_CHECK(ctx, hipMemsetAsync(a, 0, n * sizeof(double), ctx->stream));
+        return TMPG_OK;
+    }
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_zero_f64, dim3(blocks), dim3(256), 0, ctx->stream, a, n);
+    TMPG_HIP_CHECK(ctx, hipGetLastError());
+    return TMPG_OK;
+}
+
+static int rho_deposit(tmpg_ctx* ctx, double* rho, bool zero_by_kernel)
 {
     const Grid& g = ctx->g;
-    TMPG_HIP_CHECK(ctx, hipMemsetAsync(rho, 0, g.n * sizeof(double), ctx->stream));
+    int rc = zero_f64(ctx, rho, g.n, zero_by_kernel);
+    if (rc) return rc;
     for (int s = 0; s < 2; ++s) {
         Species& sp = ctx->sp[s];
         if (sp.n == 0) continue;
@@ -377,6 +400,8 @@ int diag_rho(tmpg_ctx* ctx, double* rho)
     return TMPG_OK;
 }
 
+int diag_rho(tmpg_ctx* ctx, double* rho) { return rho_deposit(ctx, rho, false); }
+
 int diag_continuity(tmpg_ctx* ctx, double* maxres, double* rhonorm)
 {
     const Grid& g = ctx->g;
@@ -396,10 +421,11 @@ int diag_continuity(tmpg_ctx* ctx, double* maxres, double* rhonorm)
     return TMPG_OK;
 }
 
-int diag_gauss(tmpg_ctx* ctx, double* rho_a, double* rho_b, int npasses, double out[4])
+int diag_gauss(tmpg_ctx* ctx, double* rho_a, double* rho_b, double* red, int npasses, bool zero_by_kernel,
+               double out[4])
 {
     const Grid& g = ctx->g;
-    int rc = diag_rho(ctx, rho_a);
+    int rc = rho_deposit(ctx, rho_a, zero_by_kernel);
     if (rc) return rc;
     // the n binomial passes the step applies to the deposited current
     double* a = rho_a;
@@ -410,8 +436,9 @@ int diag_gauss(tmpg_ctx* ctx, double* rho_a, double* rho_b, int npasses, double 
             hipLaunchKernelGGL(k_binomial_axis, dim3(blocks), dim3(256), 0, ctx->stream, a, b, g.nx, g.ny, g.nz, ax);
             std::swap(a, b);
         }
-    double* d = ctx->red + 8;
-    TMPG_HIP_CHECK(ctx, hipMemsetAsync(d, 0, 4 * sizeof(double), ctx->stream));
+    double* d = red;
+    rc = zero_f64(ctx, d, 4, zero_by_kernel);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_gauss, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[EX], ctx->f[EY], ctx->f[EZ], a, g.dim,
                        g.nx, g.ny, g.nz, d);
     TMPG_HIP_CHECK(ctx, hipGetLastError());
@@ -423,6 +450,54 @@ int diag_gauss(tmpg_ctx* ctx, double* rho_a, double* rho_b, int npasses, double 
     out[2] = h[2];
     out[3] = h[3];
     return TMPG_OK;
+}
+
+// ---------------------------------------------- allocation-free variant
+// Scratch of tmpg_gauss_residual_noalloc: two FP64 grids of at most
+// kNoAllocCells cells (the charge density and its filtered copy), then the
+// four reduction slots.  One copy per GPU, held until the process ends.  In
+// host code `scratch` names a host-side stand-in: the device address comes
+// from hipGetSymbolAddress, once per device, and only that address is used.
+constexpr int64_t kNoAllocCells = (int64_t)1 << 22;
+static __device__ __align__(256) double scratch[2 * kNoAllocCells + 32];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};   // device address of `scratch`, per device
+}  // namespace
+
+// the current device's address of the scratch array (looked up on the first
+// call on that device, with that device current)
+int noalloc_scratch(tmpg_ctx* ctx, double** out)
+{
+    int dev = 0;
+    TMPG_HIP_CHECK(ctx, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        ctx->err = "allocation-free scratch: device index out of range";
+        return TMPG_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        TMPG_HIP_CHECK(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return TMPG_OK;
+}
+
+int diag_gauss_noalloc(tmpg_ctx* ctx, int npasses, double out[4])
+{
+    if (ctx->g.n > kNoAllocCells) {
+        ctx->err = "tmpg_gauss_residual_noalloc: " + std::to_string(ctx->g.n) + " cells exceed its scratch of " +
+                   std::to_string(kNoAllocCells) + " cells";
+        return TMPG_EINVAL;
+    }
+    double* base = nullptr;
+    int rc = noalloc_scratch(ctx, &base);
+    if (rc) return rc;
+    return diag_gauss(ctx, base, base + kNoAllocCells, base + 2 * kNoAllocCells, npasses, true, out);
 }
 
 // Maxwell-Juttner speed table: inverse CDF of f(u) ~ u^2 exp(-(gamma-1)/theta)

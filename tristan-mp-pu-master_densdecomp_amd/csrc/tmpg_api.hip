@@ -255,6 +255,15 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
         tmpg_destroy(ctx);
         return TMPG_EHIP;
     }
+    {
+        // set-up of the allocation-free entry point's scratch on this device
+        double* sp = nullptr;
+        if (noalloc_scratch(ctx, &sp)) {
+            g_create_err = ctx->err;
+            tmpg_destroy(ctx);
+            return TMPG_EHIP;
+        }
+    }
     *out = ctx;
     return TMPG_OK;
 }
@@ -734,7 +743,18 @@ int tmpg_gauss_residual(tmpg_ctx* ctx, double out[4])
         rc = tmpg_model_get_base_passes(ctx->denoiser, &np);
         if (rc) { ctx->err = "tmpg_gauss_residual: denoiser base passes"; return rc; }
     }
-    return diag_gauss(ctx, ctx->rho, ctx->rho_work, np, out);
+    return diag_gauss(ctx, ctx->rho, ctx->rho_work, ctx->red + 8, np, false, out);
+}
+
+int tmpg_gauss_residual_noalloc(tmpg_ctx* ctx, double out[4])
+{
+    if (!valid_ctx(ctx) || !out) return TMPG_EINVAL;
+    int np = ctx->cfg.ntimes;
+    if (ctx->denoiser) {
+        const int rc = tmpg_model_get_base_passes(ctx->denoiser, &np);
+        if (rc) { ctx->err = "tmpg_gauss_residual_noalloc: denoiser base passes"; return rc; }
+    }
+    return diag_gauss_noalloc(ctx, np, out);
 }
 
 int tmpg_load_uniform(tmpg_ctx* ctx, int species, int ppc, float drift_gb, float theta, uint64_t seed)

@@ -166,7 +166,13 @@ int diag_energies(tmpg_ctx* ctx, double out[4]);
 int diag_rho(tmpg_ctx* ctx, double* rho_dev);
 int diag_continuity(tmpg_ctx* ctx, double* maxres, double* rhonorm);
 // Gauss's law residual with rho filtered by npasses binomial passes; rho_a
-// and rho_b are two FP64 grids of scratch
-int diag_gauss(tmpg_ctx* ctx, double* rho_a, double* rho_b, int npasses, double out[4]);
+// and rho_b are two FP64 grids of scratch, red 4 doubles of reductions;
+// zero_by_kernel clears them with a kernel instead of a memset
+int diag_gauss(tmpg_ctx* ctx, double* rho_a, double* rho_b, double* red, int npasses, bool zero_by_kernel,
+               double out[4]);
+// the same on the library's static per-GPU scratch: allocates nothing
+int diag_gauss_noalloc(tmpg_ctx* ctx, int npasses, double out[4]);
+// device address of that scratch on the current device
+int noalloc_scratch(tmpg_ctx* ctx, double** out);
 int denoise_apply(tmpg_ctx* ctx, tmpg_model* m);
 }  // namespace tmpg

@@ -30,7 +30,7 @@ EXPORTS = [
     "tmpg_set_particles", "tmpg_count_particles", "tmpg_get_particles", "tmpg_sort_particles",
     "tmpg_advance_b_half", "tmpg_advance_e", "tmpg_reset_currents", "tmpg_move", "tmpg_deposit",
     "tmpg_push_deposit", "tmpg_fold_currents", "tmpg_filter", "tmpg_add_current", "tmpg_step",
-    "tmpg_diag", "tmpg_gauss_residual", "tmpg_charge_density", "tmpg_track_continuity", "tmpg_load_uniform",
+    "tmpg_diag", "tmpg_gauss_residual", "tmpg_gauss_residual_noalloc", "tmpg_charge_density", "tmpg_track_continuity", "tmpg_load_uniform",
     "tmpg_last_timings", "tmpg_input_open", "tmpg_input_close", "tmpg_input_get_int",
     "tmpg_input_get_double",
     "tmpg_model_create", "tmpg_model_destroy", "tmpg_model_last_error", "tmpg_model_num_params",
@@ -243,6 +243,12 @@ class Context:
         """[max |div_b E - F^n rho|, rms F^n rho, max |F^n rho|, max |div_b E|]"""
         out = np.zeros(4, dtype=np.float64)
         self._chk(self.lib.tmpg_gauss_residual(self.h, _p(out)), "gauss_residual")
+        return out
+
+    def gauss_residual_noalloc(self):
+        """gauss_residual on the library's static scratch (no device allocation)"""
+        out = np.zeros(4, dtype=np.float64)
+        self._chk(self.lib.tmpg_gauss_residual_noalloc(self.h, _p(out)), "gauss_residual_noalloc")
         return out
 
     def charge_density(self):
