@@ -106,6 +106,13 @@ int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n,
                        const float* x, const float* y, const float* z,
                        const float* u, const float* v, const float* w,
                        const float* ch, const int32_t* id);
+/* the same arguments, appended to the species' particles (the reference's
+ * injections add particles, code/particles.F90:2549-2938); ids not given
+ * continue the count */
+int tmpg_add_particles(tmpg_ctx* ctx, int species, int64_t n,
+                       const float* x, const float* y, const float* z,
+                       const float* u, const float* v, const float* w,
+                       const float* ch, const int32_t* id);
 int tmpg_count_particles(tmpg_ctx* ctx, int species, int64_t* n);
 /* [reorder_particles] code/particles.F90:394-497 (a counting sort of each
  * species by cell every 10 laps, called from code/tristanmainloop.F90:303):
@@ -177,7 +184,36 @@ int tmpg_gauss_residual_noalloc(tmpg_ctx* ctx, double out[4]);
 int tmpg_charge_density(tmpg_ctx* ctx, double* rho);
 int tmpg_track_continuity(tmpg_ctx* ctx, int on);
 
-/* plasma loader (next to code/particles.F90:2549-2938) ----------------- */
+/* [inject_plasma_region] code/particles.F90:2549-2938 on the host, for one
+ * periodic domain, with the reference's own random stream: maxwell_dist
+ * (:2176-2273) on the tables of init_maxw_table (:2126-2163) and the
+ * Park-Miller random() (code/aux.F90:82-99), Poisson counts below 10
+ * expected particles (:109-134).  Fills `n` ions and n*upsamp_e electrons
+ * into ions[7*cap] and lecs[7*cap] (rows x, y, z, u, v, w, ch, each of
+ * stride cap; absolute coordinates as at tmpg_set_particles) and advances
+ * *dseed; a region of more than cap particles returns TMPG_EOVERFLOW with
+ * the count in *n and *dseed unchanged.  Seeded like the reference
+ * (123457 + rank, code/communications.F90:224-231) the positions are the
+ * reference's FP32 values; momenta agree to the math library's last bits.
+ * Host only: no context, no GPU. */
+typedef struct tmpg_plasma_region {
+    int32_t dim, order;          /* ghost layout: nghost 5 (order 1) or 7 */
+    int32_t nx, ny, nz;          /* interior cells (<grid> mx0, my0, mz0) */
+    float   c;                   /* Courant number (the cd of maxwell_dist) */
+    float   x1, x2, y1, y2, z1, z2;  /* region, the reference's global coordinates */
+    float   ppc;                 /* particles per cell, both species together */
+    float   gamma_drift;         /* drift gamma with its sign; |.| < 1 read as v/c */
+    float   delgam_i, delgam_e;  /* kT / m c^2 of ions and of electrons */
+    float   weight;              /* ch of the ions; electrons weight/upsamp_e */
+    int32_t direction;           /* drift along 1: x, 2: y, 3: z */
+    int32_t pcosthmult;          /* 2D: 1 three-dimensional spread, 0 in the plane */
+    float   sigma;               /* <particles> sigma: nonzero draws the 2D polar angle again */
+    int32_t upsamp_e;            /* electrons per ion (0 or 1: one) */
+} tmpg_plasma_region;
+int tmpg_inject_plasma_region(const tmpg_plasma_region* r, double* dseed, int64_t cap, int64_t* n,
+                              float* ions, float* lecs);
+
+/* plasma loader on the device (next to code/particles.F90:2549-2938) ---- */
 /* appends ppc particles per cell of one species, uniform positions,
  * drift gamma*beta along x (sign), isotropic thermal spread with
  * temperature theta = kT/mc^2 (Maxwell-Juttner, boosted), counter-based RNG

@@ -18,7 +18,11 @@
 !                (state-in currents are read after the fields;
 !                apply_filter2_opt in a -Dfilter2 build, as the main loop
 !                at code/tristanmainloop.F90:217-219),
-!                4 Yee only (B half, B half, E), 5 deposit + fold + filter
+!                4 Yee only (B half, B half, E), 5 deposit + fold + filter,
+!                6 initial plasma: the particles of state-in are dropped, the
+!                random seed is initialised (code/communications.F90:224-231)
+!                and the user module's init_particle_distribution_user runs
+!                (code/overload.F90:148); no steps
 !   optional   : <history-file> <every>: in mode 0, every <every> steps one
 !                line "step E_energy B_energy KE_ions KE_electrons" in FP64:
 !                0.5*sum(E^2), 0.5*sum(B^2) over the interior cells and
@@ -126,6 +130,13 @@ program tristan_ref
 	call read_species(maxhlf + 1, ne)
 	close(21)
 
+	if (mode .eq. 6) then
+		ions = 0
+		lecs = 0
+		call initialize_random_seed()
+		call init_particle_distribution_user()
+		nsteps = 0
+	endif
 	if (every .gt. 0) open(unit=23, file=trim(histfile), status='replace', action='write')
 	do step = 1, nsteps
 		select case (mode)

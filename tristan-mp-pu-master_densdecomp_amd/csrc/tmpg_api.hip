@@ -328,8 +328,11 @@ int tmpg_get_currents(tmpg_ctx* ctx, float* jx, float* jy, float* jz)
     return get_grid(ctx, JX, 3, a);
 }
 
-int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, const float* y, const float* z,
-                       const float* u, const float* v, const float* w, const float* ch, const int32_t* id)
+// particles from host arrays in the reference's absolute coordinates: they
+// replace the species' particles, or are appended to them
+static int put_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, const float* y, const float* z,
+                         const float* u, const float* v, const float* w, const float* ch, const int32_t* id,
+                         bool append)
 {
     if (!valid_ctx(ctx)) return TMPG_EINVAL;
     if (species < 0 || species > 1 || n < 0) { ctx->err = "bad species or count"; return TMPG_EINVAL; }
@@ -337,11 +340,15 @@ int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, co
         ctx->err = "null particle array";
         return TMPG_EINVAL;
     }
-    if (n > INT32_MAX && ctx->cfg.track_ids) { ctx->err = "too many particles for int32 ids"; return TMPG_EINVAL; }
-    bool need_ch = false;
+    Species& sp = ctx->sp[species];
+    const int64_t n0 = append ? sp.n : 0;
+    if (n0 + n > INT32_MAX && ctx->cfg.track_ids) { ctx->err = "too many particles for int32 ids"; return TMPG_EINVAL; }
+    bool need_ch = append && sp.has_ch;
+    bool new_ch = false;
     if (ch)
         for (int64_t p = 0; p < n; ++p)
-            if (ch[p] != 1.0f) { need_ch = true; break; }
+            if (ch[p] != 1.0f) { new_ch = true; break; }
+    need_ch = need_ch || new_ch;
     const bool ids = ctx->cfg.track_ids != 0;
     std::vector<int32_t> cell((size_t)n);
     std::vector<float> o[3];
@@ -369,39 +376,84 @@ int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, co
         if (g.dim == 2) c[2] = 0;
         cell[p] = pack_cell(ctx->pk, c[0], c[1], c[2]);
     }
+    // the species' particles so far, compacted (appending only)
+    PBuf old;
+    int64_t nold = 0;
+    int rc = TMPG_OK;
+    if (append) {
+        rc = particles_flatten(ctx, species, old, &nold);
+        if (rc) { free_pbuf(old); return rc; }
+    }
     PBuf flat;
-    int rc = alloc_pbuf(ctx, flat, std::max<int64_t>(n, 1), need_ch, ids);
-    if (rc) return rc;
+    rc = alloc_pbuf(ctx, flat, std::max<int64_t>(nold + n, 1), need_ch, ids);
+    if (rc) { free_pbuf(old); return rc; }
     auto up = [&](void* d, const void* src, size_t bytes) -> int {
         if (!bytes) return TMPG_OK;
         TMPG_HIP_CHECK(ctx, hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream));
         return TMPG_OK;
     };
+    auto dd = [&](void* d, const void* src, size_t bytes) -> int {
+        if (!bytes) return TMPG_OK;
+        TMPG_HIP_CHECK(ctx, hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return TMPG_OK;
+    };
+    const size_t fb0 = (size_t)nold * sizeof(float), ib0 = (size_t)nold * sizeof(int32_t);
+    if (nold > 0) {
+        rc |= dd(flat.cell, old.cell, ib0);
+        rc |= dd(flat.ox, old.ox, fb0);
+        rc |= dd(flat.oy, old.oy, fb0);
+        if (flat.oz) rc |= dd(flat.oz, old.oz, fb0);
+        rc |= dd(flat.u, old.u, fb0);
+        rc |= dd(flat.v, old.v, fb0);
+        rc |= dd(flat.w, old.w, fb0);
+        if (ids) rc |= dd(flat.id, old.id, ib0);
+    }
+    std::vector<float> ones;
+    if (need_ch) {
+        if (old.ch) rc |= dd(flat.ch, old.ch, fb0);
+        if (!old.ch || !new_ch) ones.assign((size_t)std::max(nold, n), 1.0f);
+        if (!old.ch) rc |= up(flat.ch, ones.data(), fb0);
+    }
     const size_t fb = (size_t)n * sizeof(float);
-    rc |= up(flat.cell, cell.data(), (size_t)n * sizeof(int32_t));
-    rc |= up(flat.ox, o[0].data(), fb);
-    rc |= up(flat.oy, o[1].data(), fb);
-    if (flat.oz) rc |= up(flat.oz, o[2].data(), fb);
-    rc |= up(flat.u, u, fb);
-    rc |= up(flat.v, v, fb);
-    rc |= up(flat.w, w, fb);
-    if (need_ch) rc |= up(flat.ch, ch, fb);
+    rc |= up(flat.cell + nold, cell.data(), (size_t)n * sizeof(int32_t));
+    rc |= up(flat.ox + nold, o[0].data(), fb);
+    rc |= up(flat.oy + nold, o[1].data(), fb);
+    if (flat.oz) rc |= up(flat.oz + nold, o[2].data(), fb);
+    rc |= up(flat.u + nold, u, fb);
+    rc |= up(flat.v + nold, v, fb);
+    rc |= up(flat.w + nold, w, fb);
+    if (need_ch) rc |= up(flat.ch + nold, new_ch ? ch : ones.data(), fb);
     std::vector<int32_t> tmp;
     if (ids) {
         const int32_t* src = id;
         if (!src) {
+            // ids continue after the particles already there
             tmp.resize((size_t)n);
-            for (int64_t p = 0; p < n; ++p) tmp[(size_t)p] = (int32_t)p;
+            for (int64_t p = 0; p < n; ++p) tmp[(size_t)p] = (int32_t)(nold + p);
             src = tmp.data();
         }
-        rc |= up(flat.id, src, (size_t)n * sizeof(int32_t));
+        rc |= up(flat.id + nold, src, (size_t)n * sizeof(int32_t));
     }
-    if (!rc) rc = particles_bin(ctx, species, flat, n, need_ch, ids);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_pbuf(old);
+    if (!rc) rc = particles_bin(ctx, species, flat, nold + n, need_ch, ids);
     (void)hipStreamSynchronize(ctx->stream);
     free_pbuf(flat);
     ctx->move_pending = false;
     ctx->have_rho_prev = false;
     return rc;
+}
+
+int tmpg_set_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, const float* y, const float* z,
+                       const float* u, const float* v, const float* w, const float* ch, const int32_t* id)
+{
+    return put_particles(ctx, species, n, x, y, z, u, v, w, ch, id, false);
+}
+
+int tmpg_add_particles(tmpg_ctx* ctx, int species, int64_t n, const float* x, const float* y, const float* z,
+                       const float* u, const float* v, const float* w, const float* ch, const int32_t* id)
+{
+    return put_particles(ctx, species, n, x, y, z, u, v, w, ch, id, true);
 }
 
 int tmpg_count_particles(tmpg_ctx* ctx, int species, int64_t* n)

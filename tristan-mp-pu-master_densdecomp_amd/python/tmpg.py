@@ -27,7 +27,7 @@ COMPAT_DEFAULT = 3
 EXPORTS = [
     "tmpg_create", "tmpg_destroy", "tmpg_last_error", "tmpg_sync",
     "tmpg_set_fields", "tmpg_get_fields", "tmpg_set_currents", "tmpg_get_currents",
-    "tmpg_set_particles", "tmpg_count_particles", "tmpg_get_particles", "tmpg_sort_particles",
+    "tmpg_set_particles", "tmpg_add_particles", "tmpg_count_particles", "tmpg_inject_plasma_region", "tmpg_get_particles", "tmpg_sort_particles",
     "tmpg_advance_b_half", "tmpg_advance_e", "tmpg_reset_currents", "tmpg_move", "tmpg_deposit",
     "tmpg_push_deposit", "tmpg_fold_currents", "tmpg_filter", "tmpg_add_current", "tmpg_step",
     "tmpg_diag", "tmpg_gauss_residual", "tmpg_gauss_residual_noalloc", "tmpg_charge_density", "tmpg_track_continuity", "tmpg_load_uniform",
@@ -66,8 +66,46 @@ class Config(ctypes.Structure):
     ]
 
 
+class PlasmaRegion(ctypes.Structure):
+    """tmpg_plasma_region: the arguments of inject_plasma_region
+    (code/particles.F90:2549-2938) and the domain they refer to"""
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("order", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("c", ctypes.c_float),
+        ("x1", ctypes.c_float), ("x2", ctypes.c_float), ("y1", ctypes.c_float), ("y2", ctypes.c_float),
+        ("z1", ctypes.c_float), ("z2", ctypes.c_float),
+        ("ppc", ctypes.c_float), ("gamma_drift", ctypes.c_float),
+        ("delgam_i", ctypes.c_float), ("delgam_e", ctypes.c_float), ("weight", ctypes.c_float),
+        ("direction", ctypes.c_int32), ("pcosthmult", ctypes.c_int32), ("sigma", ctypes.c_float),
+        ("upsamp_e", ctypes.c_int32),
+    ]
+
+
 class TmpgError(RuntimeError):
     pass
+
+
+def inject_plasma_region(region, dseed):
+    """(ions (7, n), electrons (7, n*upsamp_e), new seed): the reference's
+    plasma loader on the host (no GPU)"""
+    lib = load_library()
+    seed = ctypes.c_double(dseed)
+    n = ctypes.c_int64()
+    rc = lib.tmpg_inject_plasma_region(ctypes.byref(region), ctypes.byref(seed), ctypes.c_int64(0),
+                                       ctypes.byref(n), None, None)
+    if rc not in (0, -4):
+        raise TmpgError(f"tmpg_inject_plasma_region: {rc}")
+    cap = max(1, n.value * max(1, region.upsamp_e))
+    ions = np.zeros((7, cap), np.float32)
+    lecs = np.zeros((7, cap), np.float32)
+    seed = ctypes.c_double(dseed)
+    rc = lib.tmpg_inject_plasma_region(ctypes.byref(region), ctypes.byref(seed), ctypes.c_int64(cap),
+                                       ctypes.byref(n), _p(ions), _p(lecs))
+    if rc != 0:
+        raise TmpgError(f"tmpg_inject_plasma_region: {rc}")
+    ne = n.value * max(1, region.upsamp_e)
+    return ions[:, :n.value].copy(), lecs[:, :ne].copy(), seed.value
 
 
 _lib = None
@@ -177,6 +215,15 @@ class Context:
         idp = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
         self._chk(self.lib.tmpg_set_particles(self.h, species, ctypes.c_int64(n), *(_p(r) for r in rows),
                                               _p(idp)), "set_particles")
+
+    def add_particles(self, species, prt, ids=None):
+        """append (7, n) particles to the species (tmpg_add_particles)"""
+        prt = np.ascontiguousarray(prt, dtype=np.float32)
+        n = prt.shape[1]
+        rows = [np.ascontiguousarray(prt[r]) for r in range(7)]
+        idp = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        self._chk(self.lib.tmpg_add_particles(self.h, species, ctypes.c_int64(n), *(_p(r) for r in rows),
+                                              _p(idp)), "add_particles")
 
     def count(self, species):
         n = ctypes.c_int64()
