@@ -11,9 +11,18 @@
 !   <problem> temperature_ratio          user/user_weibel.F90:82-120
 !   <restart> irestart, intrestart, laprestart, lnamedrestartint
 !                                        code/restart.F90:95-98
-! The setup is the counter-streaming Weibel problem of
-! user/user_weibel.F90:252-292: two plasma regions drifting -x and +x, each
-! holding both species.
+! The problem (<gpu> problem, the user module the reference links):
+!   0 Weibel, user/user_weibel.F90:252-292: two plasma regions over the
+!     whole box drifting -x and +x, each holding both species;
+!   1 two-stream, user/user_twostream.F90:236-305: the same two beams with
+!     the in-plane Maxwellian (pcosthmult = 0), then the ions removed;
+!   2 plasma oscillation, user/user_plasmaosc.F90:236-305: one plasma at
+!     rest, the electrons kicked by beta sin(2 pi x / (mx0 - 5)), the ions
+!     removed (an immobile background).
+! <gpu> loader = 1 (default) draws the plasma with the reference's own
+! random stream (tmpg_inject_plasma_region, seed 123457 + <gpu> seed, like
+! 123457 + rank); 0 uses the library's device loader (tmpg_load_uniform:
+! the same distributions from a counter-based generator, Weibel only).
 !-------------------------------------------------------------------------------
 module m_host_setup
 	use, intrinsic :: iso_c_binding
@@ -25,10 +34,10 @@ module m_host_setup
 		integer :: dim = 2, order = 3
 		integer :: nx = 100, ny = 100, nz = 1
 		integer :: last = 100, ntimes = 4, highorder = 1, interval = 10
-		integer :: fused = 1, seed = 0, pusher = 0, track_ids = 0
+		integer :: fused = 1, seed = 0, pusher = 0, track_ids = 0, problem = 0, loader = 1, scheme = 0
 		real(8) :: c = 0.45d0, corr = 1.025d0
 		real(8) :: ppc0 = 1d0, c_omp = 0d0, gamma0 = 1d0, delgam = 1d-8, me = 1d0, mi = 20d0
-		real(8) :: temperature_ratio = 1d0
+		real(8) :: temperature_ratio = 1d0, sigma = 0d0
 		! normalised
 		real(8) :: qe, qi, qme, qmi, mass_e, mass_i, beta, g0
 		type(restart_params) :: rst
@@ -58,6 +67,7 @@ contains
 		call gd("particles", "me", 1d0, hp%me)
 		call gd("particles", "mi", 20d0, hp%mi)
 		call gd("problem", "temperature_ratio", 1d0, hp%temperature_ratio)
+		call gd("particles", "sigma", 0d0, hp%sigma)
 		call gi("output", "interval", 10, hp%interval)
 		! keys of this host (the shape order and dimensionality are compile-time
 		! cpp flags of the reference, code/particles_movedeposit.F90:69-84)
@@ -67,6 +77,9 @@ contains
 		call gi("gpu", "seed", 0, hp%seed)
 		call gi("gpu", "pusher", 0, hp%pusher)
 		call gi("gpu", "track_ids", 0, hp%track_ids)
+		call gi("gpu", "problem", 0, hp%problem)
+		call gi("gpu", "loader", 1, hp%loader)
+		call gi("gpu", "scheme", 0, hp%scheme)
 		call read_restart_input(inp, hp%rst)
 		hp%rst%track_ids = hp%track_ids /= 0
 		call tmpg_check(tmpg_input_close(inp), "tmpg_input_close", c_null_ptr)
@@ -127,14 +140,115 @@ contains
 		cfg%sort_every = 0
 		cfg%highorder = merge(1, 0, hp%highorder /= 0)
 		cfg%pusher = hp%pusher
-		cfg%scheme = 0
+		cfg%scheme = hp%scheme
 		cfg%reserved = 0
 	end subroutine make_config
 
-	! user/user_weibel.F90:252-292: two regions of ppc0/2 particles per cell,
-	! drifting -x and +x, each with ions and electrons (ppc0/4 per species);
-	! electron temperature delgam*mi/me*temperature_ratio
+	! init_particle_distribution_user of the problem's user module
 	subroutine init_particle_distribution_user(hp, ctx)
+		type(host_params), intent(in) :: hp
+		type(c_ptr), intent(in) :: ctx
+		real(c_double) :: dseed
+		real(c_float) :: g0, beta, delgam_i, delgam_e, ppc, x2, y2, z2
+		integer :: ng, ngz, mx0, my0, mz0
+		if (hp%problem < 0 .or. hp%problem > 2) then
+			write(*, '(a,i0)') "tristan-mp-gpu: unknown <gpu> problem ", hp%problem
+			stop 1
+		endif
+		if (hp%loader == 0) then
+			if (hp%problem /= 0) then
+				write(*, '(a)') "tristan-mp-gpu: the device loader (<gpu> loader = 0) sets up the Weibel problem only"
+				stop 1
+			endif
+			call weibel_device_loader(hp, ctx)
+			return
+		endif
+		! the reference's grid sizes include the ghost cells (code/fields.F90:186)
+		ng = merge(5, 7, hp%order == 1)
+		ngz = merge(ng, 5, hp%dim == 3)
+		mx0 = hp%nx + ng
+		my0 = hp%ny + ng
+		mz0 = merge(hp%nz + ngz, 1, hp%dim == 3)
+		! code/particles.F90:189-235 in the reference's precision
+		g0 = real(hp%gamma0, c_float)
+		if (g0 < 1.) g0 = sqrt(1. / (1. - g0**2))
+		beta = sqrt(1. - 1. / g0**2)
+		delgam_i = real(hp%delgam, c_float)
+		delgam_e = delgam_i * real(hp%mi, c_float) / real(hp%me, c_float) * real(hp%temperature_ratio, c_float)
+		dseed = 123457d0 + hp%seed   ! code/communications.F90:224-231
+		select case (hp%problem)
+		case (0)
+			! user/user_weibel.F90:252-292
+			x2 = mx0 - 1. * (ng / 2)
+			y2 = my0 - 1. * (ng / 2)
+			z2 = mz0 - 2.
+			ppc = real(hp%ppc0, c_float) / 2.
+			call region(1. * (ng / 2 + 1), x2, 1. * (ng / 2 + 1), y2, 3., z2, ppc, -g0, 1, .true., .false.)
+			call region(1. * (ng / 2 + 1), x2, 1. * (ng / 2 + 1), y2, 3., z2, ppc, g0, 1, .true., .false.)
+		case (1)
+			! user/user_twostream.F90:236-305
+			ppc = real(hp%ppc0, c_float) / 2.
+			call region(3., mx0 - 2., 3., my0 - 2., 3., mz0 - 2., ppc, -g0, 0, .false., .false.)
+			call region(3., mx0 - 2., 3., my0 - 2., 3., mz0 - 2., ppc, g0, 0, .false., .false.)
+		case (2)
+			! user/user_plasmaosc.F90:236-305
+			call region(3., mx0 - 2., 3., my0 - 2., 3., mz0 - 2., real(hp%ppc0, c_float), 0., 0, .false., .true.)
+		end select
+	contains
+		subroutine region(x1, x2r, y1, y2r, z1, z2r, ppcr, gd, pcm, keep_ions, kick)
+			real(c_float), intent(in) :: x1, x2r, y1, y2r, z1, z2r, ppcr, gd
+			integer, intent(in) :: pcm
+			logical, intent(in) :: keep_ions, kick
+			type(tmpg_plasma_region) :: r
+			real(c_float), allocatable, target :: ions(:, :), lecs(:, :)
+			real(8), parameter :: pi = 3.1415927   ! code/particles.F90:292
+			integer(c_int64_t) :: n
+			integer(c_int) :: rc
+			integer :: k
+			r%dim = hp%dim
+			r%order = hp%order
+			r%nx = hp%nx
+			r%ny = hp%ny
+			r%nz = hp%nz
+			r%c = real(hp%c, c_float)
+			r%x1 = x1; r%x2 = x2r; r%y1 = y1; r%y2 = y2r; r%z1 = z1; r%z2 = z2r
+			r%ppc = ppcr
+			r%gamma_drift = gd
+			r%delgam_i = delgam_i
+			r%delgam_e = delgam_e
+			r%weight = 1.
+			r%direction = 1
+			r%pcosthmult = pcm
+			r%sigma = real(hp%sigma, c_float)
+			r%upsamp_e = 1
+			rc = tmpg_inject_plasma_region(r, dseed, 0_c_int64_t, n, c_null_ptr, c_null_ptr)
+			if (rc /= TMPG_OK .and. rc /= TMPG_EOVERFLOW) call tmpg_check(rc, "tmpg_inject_plasma_region", ctx)
+			allocate(ions(max(n, 1_c_int64_t), 7), lecs(max(n, 1_c_int64_t), 7))
+			call tmpg_check(tmpg_inject_plasma_region(r, dseed, max(n, 1_c_int64_t), n, c_loc(ions), c_loc(lecs)), &
+				"tmpg_inject_plasma_region", ctx)
+			if (kick) then
+				! the electrons' sinusoidal kick, with the reference's promotions
+				do k = 1, int(n)
+					lecs(k, 4) = lecs(k, 4) + beta * sin(2 * pi / (mx0 - 5.) * lecs(k, 1))
+				enddo
+			endif
+			if (keep_ions) call add(0, ions, n)
+			call add(1, lecs, n)
+			deallocate(ions, lecs)
+		end subroutine region
+		subroutine add(species, a, n)
+			integer, intent(in) :: species
+			real(c_float), target, intent(in) :: a(:, :)
+			integer(c_int64_t), intent(in) :: n
+			if (n <= 0) return
+			call tmpg_check(tmpg_add_particles(ctx, int(species, c_int), n, c_loc(a(1, 1)), c_loc(a(1, 2)), &
+				c_loc(a(1, 3)), c_loc(a(1, 4)), c_loc(a(1, 5)), c_loc(a(1, 6)), c_loc(a(1, 7)), c_null_ptr), &
+				"tmpg_add_particles", ctx)
+		end subroutine add
+	end subroutine init_particle_distribution_user
+
+	! the Weibel setup on the device loader (ppc0/4 per species and region)
+	subroutine weibel_device_loader(hp, ctx)
 		type(host_params), intent(in) :: hp
 		type(c_ptr), intent(in) :: ctx
 		integer(c_int) :: ppc_half
@@ -151,6 +265,6 @@ contains
 			"load ions +x", ctx)
 		call tmpg_check(tmpg_load_uniform(ctx, 1_c_int, ppc_half, gb, th_e, int(4 * hp%seed + 4, c_int64_t)), &
 			"load electrons +x", ctx)
-	end subroutine init_particle_distribution_user
+	end subroutine weibel_device_loader
 
 end module m_host_setup

@@ -26,6 +26,20 @@ module m_tmpg
 		integer(c_int32_t) :: reserved(1)
 	end type tmpg_config
 
+	! tmpg_plasma_region: the arguments of inject_plasma_region
+	! (code/particles.F90:2549-2938) and the domain they refer to
+	type, bind(C) :: tmpg_plasma_region
+		integer(c_int32_t) :: dim, order, nx, ny, nz
+		real(c_float) :: c
+		real(c_float) :: x1, x2, y1, y2, z1, z2
+		real(c_float) :: ppc, gamma_drift, delgam_i, delgam_e, weight
+		integer(c_int32_t) :: direction, pcosthmult
+		real(c_float) :: sigma
+		integer(c_int32_t) :: upsamp_e
+	end type tmpg_plasma_region
+	integer(c_int), parameter :: TMPG_EOVERFLOW = -4
+	integer(c_int), parameter :: TMPG_SCHEME_ESIRKEPOV = 0, TMPG_SCHEME_ZIGZAG = 1
+
 	interface
 		integer(c_int) function tmpg_create(cfg, ctx) bind(C, name="tmpg_create")
 			import :: c_int, c_ptr, tmpg_config
@@ -74,6 +88,23 @@ module m_tmpg
 			integer(c_int), value :: species
 			integer(c_int64_t), value :: n
 			type(c_ptr), value :: x, y, z, u, v, w, ch, id
+		end function
+		integer(c_int) function tmpg_add_particles(ctx, species, n, x, y, z, u, v, w, ch, id) &
+				bind(C, name="tmpg_add_particles")
+			import :: c_int, c_ptr, c_int64_t
+			type(c_ptr), value :: ctx
+			integer(c_int), value :: species
+			integer(c_int64_t), value :: n
+			type(c_ptr), value :: x, y, z, u, v, w, ch, id
+		end function
+		integer(c_int) function tmpg_inject_plasma_region(r, dseed, cap, n, ions, lecs) &
+				bind(C, name="tmpg_inject_plasma_region")
+			import :: c_int, c_ptr, c_int64_t, c_double, tmpg_plasma_region
+			type(tmpg_plasma_region), intent(in) :: r
+			real(c_double), intent(inout) :: dseed
+			integer(c_int64_t), value :: cap
+			integer(c_int64_t), intent(out) :: n
+			type(c_ptr), value :: ions, lecs
 		end function
 		integer(c_int) function tmpg_get_particles(ctx, species, n_capacity, n, x, y, z, u, v, w, ch, id) &
 				bind(C, name="tmpg_get_particles")
