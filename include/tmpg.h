@@ -213,6 +213,35 @@ typedef struct tmpg_plasma_region {
 int tmpg_inject_plasma_region(const tmpg_plasma_region* r, double* dseed, int64_t cap, int64_t* n,
                               float* ions, float* lecs);
 
+/* field output file flds.tot.NNN ------------------------------------- */
+/* [output_tot] code/output.F90:2494-3875 for one periodic domain: one HDF5
+ * file of FP32 datasets over the padded nodes i = istep, 2 istep, ... <= mx
+ * (and j, k; k = 1 in 2D), dimensions (mz/istep, my/istep, mx/istep) in C
+ * order, in the order of :2604-2628:
+ *   dens densi v3x v3xi v3y v3yi [v3z v3zi] ex ey ez bx by bz jx jy jz
+ * (v3z, v3zi omitted in 2D when in_plane != 0, the reference's list for
+ * sigma = 0 and pcosthmult = 0).  dens/densi and the mean 3-velocities
+ * v3* follow meanq_fld_cur (:5229-5482: boxes of +-2 cells, +-0 in z in 2D,
+ * folded like exchange_current); E and B are interpolfld (:4995-5127) at
+ * the nodes; J is the current array.  The HDF5 C library is loaded at the
+ * first call (TMPG_HDF5_LIB, else libhdf5.so from the loader's path, else
+ * /opt/conda/lib/libhdf5.so); without it: TMPG_EINVAL. */
+typedef struct tmpg_flds_tot_state {
+    int32_t dim, order;          /* ghost layout as in tmpg_config */
+    int32_t nx, ny, nz;          /* interior cells */
+    int32_t istep;               /* <output> istep */
+    int32_t in_plane;            /* 2D: the 15-dataset list */
+    const float *ex, *ey, *ez, *bx, *by, *bz, *jx, *jy, *jz;   /* padded (mx,my,mz) */
+    int64_t ni, ne;              /* particles of each species */
+    const float* ions;           /* rows x, y, z, u, v, w, ch of stride ni */
+    const float* lecs;           /* the same, stride ne */
+} tmpg_flds_tot_state;
+/* host only: writes the state given (errbuf, if not NULL, receives the
+ * message of a failure) */
+int tmpg_flds_tot_write(const char* path, const tmpg_flds_tot_state* st, char* errbuf, int errlen);
+/* the context's fields, currents and particles (copied to the host) */
+int tmpg_write_flds_tot(tmpg_ctx* ctx, const char* path, int istep, int in_plane);
+
 /* plasma loader on the device (next to code/particles.F90:2549-2938) ---- */
 /* appends ppc particles per cell of one species, uniform positions,
  * drift gamma*beta along x (sign), isotropic thermal spread with

@@ -35,6 +35,9 @@ module m_host_setup
 		integer :: nx = 100, ny = 100, nz = 1
 		integer :: last = 100, ntimes = 4, highorder = 1, interval = 10
 		integer :: fused = 1, seed = 0, pusher = 0, track_ids = 0, problem = 0, loader = 1, scheme = 0
+		! flds.tot output (code/output.F90:143-149, 2546-2556): written when
+		! flds_tot /= 0, every interval laps from pltstart, sampled every istep
+		integer :: istep = 2, pltstart = 0, flds_tot = 0, pcosthmult = 1
 		real(8) :: c = 0.45d0, corr = 1.025d0
 		real(8) :: ppc0 = 1d0, c_omp = 0d0, gamma0 = 1d0, delgam = 1d-8, me = 1d0, mi = 20d0
 		real(8) :: temperature_ratio = 1d0, sigma = 0d0
@@ -69,6 +72,8 @@ contains
 		call gd("problem", "temperature_ratio", 1d0, hp%temperature_ratio)
 		call gd("particles", "sigma", 0d0, hp%sigma)
 		call gi("output", "interval", 10, hp%interval)
+		call gi("output", "istep", 2, hp%istep)
+		call gi("output", "pltstart", 0, hp%pltstart)
 		! keys of this host (the shape order and dimensionality are compile-time
 		! cpp flags of the reference, code/particles_movedeposit.F90:69-84)
 		call gi("gpu", "dim", merge(3, 2, hp%nz > 1), hp%dim)
@@ -80,6 +85,10 @@ contains
 		call gi("gpu", "problem", 0, hp%problem)
 		call gi("gpu", "loader", 1, hp%loader)
 		call gi("gpu", "scheme", 0, hp%scheme)
+		call gi("gpu", "flds_tot", 0, hp%flds_tot)
+		! the user files' pcosthmult: 1 in user_weibel, 0 in user_twostream and
+		! user_plasmaosc (it selects the 2D flds.tot dataset list)
+		hp%pcosthmult = merge(1, 0, hp%problem == 0)
 		call read_restart_input(inp, hp%rst)
 		hp%rst%track_ids = hp%track_ids /= 0
 		call tmpg_check(tmpg_input_close(inp), "tmpg_input_close", c_null_ptr)
@@ -102,6 +111,31 @@ contains
 			v = dv
 		end subroutine gd
 	end subroutine read_host_input
+
+	! output_tot (code/output.F90:2494-3875) for this lap: output/flds.tot.NNN
+	! with NNN = (lap - pltstart) / interval
+	subroutine maybe_write_flds_tot(hp, ctx, lap)
+		type(host_params), intent(in) :: hp
+		type(c_ptr), intent(in) :: ctx
+		integer, intent(in) :: lap
+		character(len=64) :: path
+		integer(c_int) :: rc
+		logical :: in_plane
+		interface
+			integer(c_int) function c_mkdir(p, mode) bind(C, name="mkdir")
+				import :: c_int, c_char
+				character(kind=c_char), intent(in) :: p(*)
+				integer(c_int), value :: mode
+			end function
+		end interface
+		if (hp%flds_tot == 0 .or. lap < hp%pltstart) return
+		if (mod(lap - hp%pltstart, hp%interval) /= 0) return
+		rc = c_mkdir(cstr("output"), int(o'755', c_int))   ! an existing directory is fine
+		write(path, '(a,i3.3)') "output/flds.tot.", (lap - hp%pltstart) / hp%interval
+		in_plane = hp%dim == 2 .and. hp%sigma == 0d0 .and. hp%pcosthmult == 0
+		call tmpg_check(tmpg_write_flds_tot(ctx, cstr(path), int(hp%istep, c_int), merge(1_c_int, 0_c_int, in_plane)), &
+			"tmpg_write_flds_tot "//trim(path), ctx)
+	end subroutine maybe_write_flds_tot
 
 	! code/particles.F90:206-235
 	subroutine normalise(hp)

@@ -172,6 +172,12 @@ module m_tmpg
 			type(c_ptr), value :: ctx
 			real(c_double), intent(out) :: out(4)
 		end function
+		integer(c_int) function tmpg_write_flds_tot(ctx, path, istep, in_plane) bind(C, name="tmpg_write_flds_tot")
+			import :: c_int, c_ptr, c_char
+			type(c_ptr), value :: ctx
+			character(kind=c_char), intent(in) :: path(*)
+			integer(c_int), value :: istep, in_plane
+		end function
 		integer(c_int) function tmpg_load_uniform(ctx, species, ppc, drift_gb, theta, seed) &
 				bind(C, name="tmpg_load_uniform")
 			import :: c_int, c_ptr, c_float, c_int64_t

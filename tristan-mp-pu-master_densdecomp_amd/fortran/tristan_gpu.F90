@@ -71,7 +71,11 @@ program tristan_gpu
 	! mainloop (code/tristanmainloop.F90:107-344)
 	call system_clock(c0, crate)
 	tacc = 0d0
+	! Diagnostics() runs at lap 0 and at the top of every lap, before its
+	! step (code/tristanmainloop.F90:102-109); a resumed run skips lap 0
+	if (lapst == 1) call maybe_write_flds_tot(hp, ctx, 0)
 	do lap = lapst, hp%last
+		call maybe_write_flds_tot(hp, ctx, lap)
 		if (hp%fused /= 0) then
 			call tmpg_check(tmpg_step(ctx, 1_c_int), "tmpg_step", ctx)
 		else

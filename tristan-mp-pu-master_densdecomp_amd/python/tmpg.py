@@ -37,7 +37,7 @@ EXPORTS = [
     "tmpg_model_get_params", "tmpg_model_set_params", "tmpg_model_set_scale", "tmpg_model_get_scale",
     "tmpg_model_set_base_passes", "tmpg_model_get_base_passes", "tmpg_model_set_amplitude_weight",
     "tmpg_model_train_batch", "tmpg_model_eval", "tmpg_model_add_sample", "tmpg_model_train",
-    "tmpg_denoise", "tmpg_set_denoiser",
+    "tmpg_denoise", "tmpg_set_denoiser", "tmpg_flds_tot_write", "tmpg_write_flds_tot",
 ]
 
 
@@ -84,6 +84,43 @@ class PlasmaRegion(ctypes.Structure):
 
 class TmpgError(RuntimeError):
     pass
+
+
+class FldsTotState(ctypes.Structure):
+    """tmpg_flds_tot_state: what output_tot (code/output.F90:2494-3875)
+    samples, on the padded reference layout"""
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("order", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("istep", ctypes.c_int32), ("in_plane", ctypes.c_int32),
+    ] + [(k, ctypes.c_void_p) for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz")] + [
+        ("ni", ctypes.c_int64), ("ne", ctypes.c_int64),
+        ("ions", ctypes.c_void_p), ("lecs", ctypes.c_void_p),
+    ]
+
+
+FLDS_TOT_DATASETS = ["dens", "densi", "v3x", "v3xi", "v3y", "v3yi", "v3z", "v3zi",
+                     "ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz"]
+
+
+def flds_tot_write(path, dim, order, nx, ny, nz, istep, fields, currents, ions, lecs, in_plane=False):
+    """write flds.tot from host arrays (no GPU): fields (6, ...) and
+    currents (3, ...) on the padded grid (mz, my, mx) C order, particles
+    (7, n) rows x, y, z, u, v, w, ch"""
+    lib = load_library()
+    mx, my, mz, _, _ = extent(dim, order, nx, ny, nz)
+    arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(mz, my, mx) for a in list(fields) + list(currents)]
+    prt = [np.ascontiguousarray(a, dtype=np.float32) for a in (ions, lecs)]
+    st = FldsTotState(dim=dim, order=order, nx=nx, ny=ny, nz=nz, istep=istep, in_plane=int(bool(in_plane)))
+    for k, a in zip(("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz"), arrs):
+        setattr(st, k, a.ctypes.data)
+    st.ni, st.ne = prt[0].shape[1], prt[1].shape[1]
+    st.ions = prt[0].ctypes.data if st.ni else None
+    st.lecs = prt[1].ctypes.data if st.ne else None
+    err = ctypes.create_string_buffer(256)
+    rc = lib.tmpg_flds_tot_write(os.fsencode(path), ctypes.byref(st), err, ctypes.c_int(256))
+    if rc != 0:
+        raise TmpgError(f"tmpg_flds_tot_write: {err.value.decode()} ({rc})")
 
 
 def inject_plasma_region(region, dseed):
@@ -297,6 +334,11 @@ class Context:
         out = np.zeros(4, dtype=np.float64)
         self._chk(self.lib.tmpg_gauss_residual_noalloc(self.h, _p(out)), "gauss_residual_noalloc")
         return out
+
+    def write_flds_tot(self, path, istep=1, in_plane=False):
+        """the reference's flds.tot file of the current state"""
+        self._chk(self.lib.tmpg_write_flds_tot(self.h, os.fsencode(path), ctypes.c_int(istep),
+                                               ctypes.c_int(int(bool(in_plane)))), "write_flds_tot")
 
     def charge_density(self):
         out = np.zeros((self.nz, self.ny, self.nx), dtype=np.float64)
