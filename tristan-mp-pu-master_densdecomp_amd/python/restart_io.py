@@ -12,63 +12,92 @@ Each file is one Fortran-unformatted sequential record (code/output.F90:
              the ions followed by the electrons
 
 A record is framed by its byte length as a 4-byte integer before and after
-it.  Records longer than 2**31 - 1 bytes are split into subrecords whose
-leading and trailing lengths are negative on every subrecord but the last
-(gfortran's convention); the reader accepts both forms and
-the writer splits only when it must.
+it.  A record longer than 2**31 - 9 bytes is split into subrecords of at most
+that many bytes (the convention of gfortran and Intel Fortran): the leading
+length of a subrecord is negative when another subrecord follows, the
+trailing length negative when another precedes.  The reader also takes the
+single record whose two lengths hold its byte count modulo 2**32, the form
+LLVM flang's runtime writes beyond 2 GiB (readable by that runtime only).
 
 Used for fixture exchange with the reference and to inspect the Fortran
 host's checkpoints; the library itself does not read these files.
 """
+import os
+
 import numpy as np
 
-_MAXSUB = 2 ** 31 - 1
+_MAXSUB = 2 ** 31 - 9
 
 FIELD_NAMES = ("ex", "ey", "ez", "bx", "by", "bz")
 PART_REALS = ("x", "y", "z", "u", "v", "w", "ch")
 PART_INTS = ("ind", "proc", "splitlev")
 
 
-def read_record(path):
-    """the first record of a sequential unformatted file, as bytes"""
-    with open(path, "rb") as f:
-        data = f.read()
-    out = []
-    pos = 0
-    while True:
-        if pos + 4 > len(data):
-            raise ValueError(f"{path}: truncated record marker at byte {pos}")
-        n = int(np.frombuffer(data, "<i4", 1, pos)[0])
-        size = abs(n)
-        end = pos + 4 + size
-        if end + 4 > len(data):
-            raise ValueError(f"{path}: record of {size} bytes runs past the end of the file")
-        tail = int(np.frombuffer(data, "<i4", 1, end)[0])
-        if abs(tail) != size:
-            raise ValueError(f"{path}: leading marker {n} and trailing marker {tail} differ")
-        out.append(data[pos + 4:end])
+def _pieces(mm, path):
+    """(offset, length) of the payload pieces of the file's one record"""
+    size = len(mm)
+    pieces, pos = [], 0
+    while pos + 8 <= size:
+        n = int(np.frombuffer(mm, "<i4", 1, pos)[0])
+        length = abs(n)
+        end = pos + 4 + length
+        if end + 4 > size or abs(int(np.frombuffer(mm, "<i4", 1, end)[0])) != length:
+            break
+        pieces.append((pos + 4, length))
         pos = end + 4
         if n >= 0:
-            break
-    return b"".join(out)
+            return pieces
+    # one record whose markers hold its length modulo 2**32
+    if size >= 8:
+        lead = int(np.frombuffer(mm, "<i4", 1, 0)[0])
+        tail = int(np.frombuffer(mm, "<i4", 1, size - 4)[0])
+        if lead == tail and (size - 8) % 2 ** 32 == lead % 2 ** 32:
+            return [(4, size - 8)]
+    raise ValueError(f"{path}: not a sequential unformatted record")
+
+
+def read_record(path):
+    """the first record of a sequential unformatted file, as a bytes-like
+    object (a view of the memory-mapped file when the record is one piece)"""
+    import mmap
+    with open(path, "rb") as f:
+        if os.fstat(f.fileno()).st_size == 0:
+            raise ValueError(f"{path}: empty file")
+        mm = mmap.mmap(f.fileno(), 0, access=mmap.ACCESS_READ)
+    pieces = _pieces(mm, path)
+    if len(pieces) == 1:
+        off, length = pieces[0]
+        return memoryview(mm)[off:off + length]
+    return b"".join(mm[off:off + length] for off, length in pieces)
 
 
 def write_record(path, payload):
-    """one sequential unformatted record holding payload (bytes)"""
+    """one sequential unformatted record holding payload (bytes-like, or a
+    list of bytes-like parts written in order)"""
+    parts = payload if isinstance(payload, (list, tuple)) else [payload]
+    views = [memoryview(p).cast("B") for p in parts]
+    n = sum(len(v) for v in views)
     with open(path, "wb") as f:
-        n = len(payload)
         if n == 0:
             f.write(np.int32(0).tobytes() * 2)
             return
-        pos = 0
-        while pos < n:
-            size = min(_MAXSUB, n - pos)
-            last = pos + size == n
-            m = np.int32(size if last else -size).tobytes()
-            f.write(m)
-            f.write(payload[pos:pos + size])
-            f.write(m)
-            pos += size
+        done, first = 0, True
+        it = iter(views)
+        cur, cpos = next(it), 0
+        while done < n:
+            size = min(_MAXSUB, n - done)
+            f.write(np.int32(-size if done + size < n else size).tobytes())
+            left = size
+            while left:
+                while cpos == len(cur):
+                    cur, cpos = next(it), 0
+                take = min(left, len(cur) - cpos)
+                f.write(cur[cpos:cpos + take])
+                cpos += take
+                left -= take
+            done += size
+            f.write(np.int32(size if first else -size).tobytes())
+            first = False
 
 
 class _Cursor:
@@ -126,7 +155,7 @@ def write_fields(path, fields, lap, dseed=123457.0, xinject=0.0, xinject2=0.0, x
     parts += [np.array([dseed], "<f8").tobytes(), np.array([lap], "<i4").tobytes(),
               np.array([xinject, xinject2, xinject3], "<f8").tobytes(), np.array([leftwall], "<f4").tobytes(),
               np.array([walloc], "<f8").tobytes()]
-    write_record(path, b"".join(parts))
+    write_record(path, parts)
 
 
 def read_particles(path):
@@ -163,4 +192,4 @@ def write_particles(path, ions, lecs, maxptl=None, maxhlf=None, totalpartnum=Non
                 if a.shape != (n,):
                     raise ValueError(f"{k} has shape {a.shape}, expected ({n},)")
                 parts.append(a.tobytes())
-    write_record(path, b"".join(parts))
+    write_record(path, parts)
