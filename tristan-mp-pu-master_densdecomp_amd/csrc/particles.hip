@@ -281,12 +281,12 @@ __global__ void __launch_bounds__(512) k_push_tiled(TiledArgs a)
 // LDS instead of depositing it, unless its y/z shape window leaves the
 // 4-node window [-1,2] (a y or z cell crossing), in which case it deposits at
 // once.  Phase B counting-sorts the records by cell.  Phase C gives every
-// cell to two threads (split in z), which sum the Esirkepov stencils of all
-// of the cell's particles in registers -- x window [-2,3] (6 nodes, x
+// cell to four threads (one per z slot), which sum the Esirkepov stencils of
+// all of the cell's particles in registers -- x window [-2,3] (6 nodes, x
 // crossings included), y/z windows [-1,2] -- and add the sums to the tile's
 // LDS currents once.  That replaces ~300 LDS atomics per particle by ~250 per
 // cell.
-constexpr int CELLS_T = 512;        // threads per workgroup
+constexpr int CELLS_T = 1024;       // threads per workgroup
 constexpr int CELLS_R = 2816;       // records per pass (both species)
 
 struct CellsLds {
@@ -399,67 +399,82 @@ __device__ __forceinline__ bool fits4(float o, float d)
     return lo >= -1 && hi <= 2;
 }
 
+// lane S of every group of four lanes, to all four (DPP quad permutation)
+template <int S>
+__device__ __forceinline__ float quad_bcast(float v)
+{
+    constexpr int ctrl = S | (S << 2) | (S << 4) | (S << 6);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, true));
+}
+
 // Per-cell register sums of one cell's records (idx[r0 .. r0+n) index the
-// records) over the z slots of half HALF, added to the tile's LDS currents.
-template <int ORD, int HALF>
-__device__ __forceinline__ void cell_sums(int c, int r0, int n, int tx, int ty, const uint16_t* __restrict__ idx,
-                                          const float* __restrict__ rec, const float* __restrict__ rq,
-                                          const LdsSink& snk)
+// records) at z window slot Q (0..3) of the cell: Jx and Jy at z node slot Q,
+// Jz at prefix slot Q (slots 0..2; Q = 3 has no Jz), added to the tile's LDS
+// currents once.  The four slots of a cell are four adjacent lanes (a quad)
+// that walk the cell's records together: lane Q computes the shape profile
+// of axis Q (x, y, z; lane 3 repeats x) and the quad shares the three
+// profiles by DPP broadcasts, so each record's shapes are computed once.
+// A lane holds 62 sums, which keeps the kernel in 128 VGPRs (four waves per
+// SIMD).  The y and z profiles come from the 6-node window, whose nodes
+// -1..2 are the 4-node window of a record that keeps its y/z piece.
+template <int ORD>
+__device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, int ty,
+                                          const uint16_t* __restrict__ idx, const float* __restrict__ rec,
+                                          const float* __restrict__ rq, const LdsSink& snk)
 {
     const float twelfth = 1.0f / 12.0f;
     const int r1 = r0 + n;
     if (r1 == r0) return;
     const int li = c % tx, lj = (c / tx) % ty, lk = c / (tx * ty);
-    // z nodes of this half: Jx, Jy use z window slots {0,1} or {2,3};
-    // Jz prefix slots {0,1} or {2}
-    constexpr int zb = HALF ? 2 : 0;
-    float ax[5][4][2], ay[6][3][2], az[6][4][2];
+    const bool has_z = Q < 3;
+    const int axis = Q == 3 ? 0 : Q;
+    float ax[5][4], ay[6][3], az[6][4];
 #pragma unroll
     for (int u = 0; u < 5; ++u)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) { ax[u][b][0] = 0.0f; ax[u][b][1] = 0.0f; }
+        for (int b = 0; b < 4; ++b) ax[u][b] = 0.0f;
 #pragma unroll
     for (int u = 0; u < 6; ++u)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) { ay[u][b][0] = 0.0f; ay[u][b][1] = 0.0f; }
+        for (int b = 0; b < 3; ++b) ay[u][b] = 0.0f;
 #pragma unroll
     for (int u = 0; u < 6; ++u)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) { az[u][b][0] = 0.0f; az[u][b][1] = 0.0f; }
-    // the next record is read while this one is summed
-    float nr[7];
-    auto fetch = [&](int rr) {
-        const int r = idx[rr];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) nr[m] = rec[m * CELLS_R + r];
-        nr[6] = rq[r];
-    };
-    fetch(r0);
+        for (int b = 0; b < 4; ++b) az[u][b] = 0.0f;
     for (int rr = r0; rr < r1; ++rr) {
-        const float ox = nr[0], oy = nr[1], oz = nr[2];
-        const float dx = nr[3], dy = nr[4], dz = nr[5];
-        const float q = nr[6];
-        if (rr + 1 < r1) fetch(rr + 1);
-        float dSx[6], Ax[6], dSy[4], Ay[4], dSz[4], Az[4];
-        {
-            float S6[6];
-            profile6<ORD>(ox, dx, S6, dSx);
+        const int r = idx[rr];
+        const float o = rec[axis * CELLS_R + r], d = rec[(3 + axis) * CELLS_R + r];
+        const float q = rq[r];
+        float S6[6], dS6[6];
+        profile6<ORD>(o, d, S6, dS6);
+        float dSx[6], Ax[6], dSy[4], Ay[4];
 #pragma unroll
-            for (int u = 0; u < 6; ++u) Ax[u] = S6[u] + 0.5f * dSx[u];
+        for (int u = 0; u < 6; ++u) {
+            const float sx = quad_bcast<0>(S6[u]);
+            dSx[u] = quad_bcast<0>(dS6[u]);
+            Ax[u] = sx + 0.5f * dSx[u];
         }
-        {
-            float S4[4];
-            profile4<ORD>(oy, dy, S4, dSy);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) Ay[u] = S4[u] + 0.5f * dSy[u];
+        for (int b = 0; b < 4; ++b) {
+            const float sy = quad_bcast<1>(S6[1 + b]);
+            dSy[b] = quad_bcast<1>(dS6[1 + b]);
+            Ay[b] = sy + 0.5f * dSy[b];
         }
+        float azc, dzc, czc;
         {
-            float S4[4];
-            profile4<ORD>(oz, dz, S4, dSz);
+            float Sz[4], dSz[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) Az[u] = S4[u] + 0.5f * dSz[u];
+            for (int b = 0; b < 4; ++b) {
+                Sz[b] = quad_bcast<2>(S6[1 + b]);
+                dSz[b] = quad_bcast<2>(dS6[1 + b]);
+            }
+            const float s0 = dSz[0], s1 = s0 + dSz[1], s2 = s1 + dSz[2];
+            const float S = Q == 0 ? Sz[0] : Q == 1 ? Sz[1] : Q == 2 ? Sz[2] : Sz[3];
+            dzc = Q == 0 ? dSz[0] : Q == 1 ? dSz[1] : Q == 2 ? dSz[2] : dSz[3];
+            azc = S + 0.5f * dzc;
+            czc = q * (Q == 0 ? s0 : Q == 1 ? s1 : s2);
         }
-        float Cx[5], Cy[3], Cz[3];
+        float Cx[5], Cy[3];
         {
             float run = 0.0f;
 #pragma unroll
@@ -467,67 +482,50 @@ __device__ __forceinline__ void cell_sums(int c, int r0, int n, int tx, int ty, 
             run = 0.0f;
 #pragma unroll
             for (int u = 0; u < 3; ++u) { run += dSy[u]; Cy[u] = q * run; }
-            run = 0.0f;
-#pragma unroll
-            for (int u = 0; u < 3; ++u) { run += dSz[u]; Cz[u] = q * run; }
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int zc = zb + h;       // z window slot 0..3
-            const float azc = HALF ? Az[2 + h] : Az[h];
-            const float dzc = HALF ? dSz[2 + h] : dSz[h];
-            (void)zc;
+        for (int b = 0; b < 4; ++b) {
+            const float wyz = Ay[b] * azc + twelfth * dSy[b] * dzc;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const float wyz = Ay[b] * azc + twelfth * dSy[b] * dzc;
+            for (int u = 0; u < 5; ++u) ax[u][b] += Cx[u] * wyz;
+        }
 #pragma unroll
-                for (int u = 0; u < 5; ++u) ax[u][b][h] += Cx[u] * wyz;
-            }
+        for (int u = 0; u < 6; ++u) {
+            const float wxz = Ax[u] * azc + twelfth * dSx[u] * dzc;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) ay[u][b] += Cy[b] * wxz;
+        }
+        if (has_z) {
+            // Jz: czc (Ax Ay + dSx dSy / 12) as two rank-one updates
+            const float czt = twelfth * czc;
 #pragma unroll
             for (int u = 0; u < 6; ++u) {
-                const float wxz = Ax[u] * azc + twelfth * dSx[u] * dzc;
+                const float ga = czc * Ax[u], gd = czt * dSx[u];
 #pragma unroll
-                for (int b = 0; b < 3; ++b) ay[u][b][h] += Cy[b] * wxz;
-            }
-            // Jz prefix slot: HALF 0 -> {0,1}, HALF 1 -> {2} only
-            if (HALF == 0 || h == 0) {
-                const float czc = HALF ? Cz[2] : Cz[h];
-#pragma unroll
-                for (int u = 0; u < 6; ++u)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b)
-                        az[u][b][h] += czc * (Ax[u] * Ay[b] + twelfth * dSx[u] * dSy[b]);
+                for (int b = 0; b < 4; ++b) az[u][b] += ga * Ay[b] + gd * dSy[b];
             }
         }
     }
     // add the cell's sums to the tile currents: node (li-2+u, lj-1+b,
-    // lk-1+zslot) sits at LDS (li+u, lj+1+b, lk+1+zslot)
+    // lk-1+Q) sits at LDS (li+u, lj+1+b, lk+1+Q)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int zs = zb + h;
+    for (int b = 0; b < 4; ++b) {
+        const int row = snk.row(lj + 1 + b, lk + 1 + Q);
+#pragma unroll
+        for (int u = 0; u < 5; ++u) snk.add(0, row, li + u, ax[u][b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const int row = snk.row(lj + 1 + b, lk + 1 + Q);
+#pragma unroll
+        for (int u = 0; u < 6; ++u) snk.add(1, row, li + u, ay[u][b]);
+    }
+    if (has_z) {
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int row = snk.row(lj + 1 + b, lk + 1 + zs);
+            const int row = snk.row(lj + 1 + b, lk + 1 + Q);
 #pragma unroll
-            for (int u = 0; u < 5; ++u)
-                snk.add(0, row, li + u, ax[u][b][h]);
-        }
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            const int row = snk.row(lj + 1 + b, lk + 1 + zs);
-#pragma unroll
-            for (int u = 0; u < 6; ++u)
-                snk.add(1, row, li + u, ay[u][b][h]);
-        }
-        if (HALF == 0 || h == 0) {
-            const int zsz = HALF ? 2 : h;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int row = snk.row(lj + 1 + b, lk + 1 + zsz);
-#pragma unroll
-                for (int u = 0; u < 6; ++u)
-                    snk.add(2, row, li + u, az[u][b][h]);
-            }
+            for (int u = 0; u < 6; ++u) snk.add(2, row, li + u, az[u][b]);
         }
     }
 }
@@ -659,12 +657,12 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         }
         __syncthreads();
 
-        // ---- phase C: per-cell register sums, two threads per cell (z halves);
-        // the half is uniform per wave, so each half is its own code path
-        for (int wk = tid; wk < 2 * nc; wk += NT) {
-            const int c = perm[wk % nc];
-            if (wk < nc) cell_sums<ORD, 0>(c, start[c], cnt[c], TX, TY, idx, rec, rq, snk);
-            else cell_sums<ORD, 1>(c, start[c], cnt[c], TX, TY, idx, rec, rq, snk);
+        // ---- phase C: per-cell register sums, four adjacent lanes per cell
+        // (one per z slot)
+        static_assert(NT % 4 == 0, "quads of lanes");
+        for (int wk = tid; wk < 4 * nc; wk += NT) {
+            const int c = perm[wk >> 2];
+            cell_sums<ORD>(wk & 3, c, start[c], cnt[c], TX, TY, idx, rec, rq, snk);
         }
         __syncthreads();
         for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
@@ -692,7 +690,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             ox = p.ox; oy = p.oy; oz = p.oz;
             {
                 float E[6];
-                gather3_lds<ORD, FX, FY>(sf, (a.flags & PUSH_GATHER2) != 0, li + 1, lj + 1, lk + 1, p.ox, p.oy,
+                gather3_lds<ORD, FX, FY, 1>(sf, (a.flags & PUSH_GATHER2) != 0, li + 1, lj + 1, lk + 1, p.ox, p.oy,
                                          p.oz, E);
                 push_fields<VAY>(E, a.flags, a.c, a.cinv, sp.qm, p, dx, dy, dz);
             }
@@ -1390,6 +1388,16 @@ int particles_check(tmpg_ctx* ctx)
     return TMPG_OK;
 }
 
+// the fused push of this context runs the cell-sum kernel (3D, Esirkepov,
+// default 8x8x4 tile)
+static bool push_cell_sums(const tmpg_ctx* ctx)
+{
+    const Tiling& tl = ctx->tl;
+    if (ctx->g.dim != 3 || ctx->zz || !ctx->use_cell_sums || tl.tx != 8 || tl.ty != 8 || tl.tz != 4) return false;
+    return cells_lds(tl.tx + 3, tl.ty + 3, tl.tz + 3, tl.tx + 5, tl.ty + 5, tl.tz + 5, tl.tx * tl.ty * tl.tz).bytes <=
+           160 * 1024;
+}
+
 int particles_push(tmpg_ctx* ctx, int mode)
 {
     const Grid& g = ctx->g;
@@ -1448,8 +1456,7 @@ int particles_push(tmpg_ctx* ctx, int mode)
         // the cell-sum kernel is compiled for the default 8x8x4 tile; other
         // tilings (explicit, or clamped on grids under 8x8x4 cells) take the
         // general tiled kernel
-        const bool cells = prim && ctx->use_cell_sums && tl.tx == 8 && tl.ty == 8 && tl.tz == 4 &&
-                           CL.bytes <= 160 * 1024;
+        const bool cells = push_cell_sums(ctx);
         (void)ncell_tile;
         const bool vay = ctx->cfg.pusher == 1;
         // order 3 only: there a direct deposit (about 300 LDS atomics per

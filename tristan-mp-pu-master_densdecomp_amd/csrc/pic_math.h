@@ -193,44 +193,54 @@ __device__ __forceinline__ int axis_deposit(float o, float dl, float* S1, float*
 }
 
 // ------------------------------------------------------- gather weights
-// Weights of nodes -1..2 relative to cell i for a particle at offset o,
-// primal (Sp, at the cell nodes) and dual (Sd, relative to cell id = i or
-// i-1, at offset o -+ 1/2), as the movers compute them.  For order 2 with
-// `quirk` the primal weights are masked to the dual's slot range, as the
-// reference's loops do (code/particles_movedeposit.F90:709-832).
+// weights of a particle at offset X in [0,1) on the 4-slot frame of nodes
+// -1..2; returns the piece.  For X in [0,1) the piece is floor(X) = 0 for
+// orders 1 and 3 and ceil(X - 1/2) in {0, 1} for order 2, so the placement
+// is a fixed slot assignment (orders 1, 3) or one select (order 2).
 template <int ORD>
-__device__ __forceinline__ void gather_weights(float o, bool quirk, float* Sp, float* Sd, int& did)
+__device__ __forceinline__ int frame_weights(float X, float* S)
 {
     using P = Piece<ORD>;
     float t[P::cnt];
+    if constexpr (ORD == 2) {
+        const bool up = X > 0.5f;          // piece_of<2>(X) == 1
+        P::w(X - (up ? 1.0f : 0.0f), t);
+        S[0] = up ? 0.0f : t[0];
+        S[1] = up ? t[0] : t[1];
+        S[2] = up ? t[1] : t[2];
+        S[3] = up ? t[2] : 0.0f;
+        return up ? 1 : 0;
+    } else {
+        P::w(X, t);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) { Sp[k] = 0.0f; Sd[k] = 0.0f; }
-    // primal
-    const int mp = piece_of<ORD>(o);
-    P::w(o - (float)mp, t);
-#pragma unroll
-    for (int k = 0; k < P::cnt; ++k)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (q == mp + P::first + k + 1) Sp[q] = t[k];
+        for (int q = 0; q < 4; ++q) {
+            const int k = q - 1 - P::first;     // slot q holds node q - 1 = first + k
+            S[q] = (k >= 0 && k < P::cnt) ? t[k] : 0.0f;
+        }
+        return 0;
+    }
+}
+
+// Weights of nodes -1..2 relative to cell i for a particle at offset o in
+// [0,1), primal (Sp, at the cell nodes) and dual (Sd, relative to cell
+// id = i + did, did = 0 if o >= 1/2 else -1, at offset o -+ 1/2), as the
+// movers compute them.  For order 2 with `quirk` the primal weights are
+// masked to the dual's slot range, as the reference's loops do
+// (code/particles_movedeposit.F90:709-832).
+template <int ORD>
+__device__ __forceinline__ void gather_weights(float o, bool quirk, float* Sp, float* Sd, int& did)
+{
+    frame_weights<ORD>(o, Sp);
     // dual: id = i if o >= 1/2 else i - 1
-    float od;
-    if (o >= 0.5f) { did = 0; od = o - 0.5f; } else { did = -1; od = o + 0.5f; }
-    const int md = piece_of<ORD>(od);
-    P::w(od - (float)md, t);
-#pragma unroll
-    for (int k = 0; k < P::cnt; ++k)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (q == md + P::first + k + 1) Sd[q] = t[k];
+    const bool hi = o >= 0.5f;
+    did = hi ? 0 : -1;
+    const float od = hi ? o - 0.5f : o + 0.5f;
+    const int md = frame_weights<ORD>(od, Sd);
     if (ORD == 2 && quirk) {
         // the reference loops over the dual's slots: nodes (md-1 .. md+1)
         // of the 6-slot frame, applied with the primal's cell origin
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int node = k - 1;
-            if (node < md - 1 || node > md + 1) Sp[k] = 0.0f;
-        }
+        if (md == 0) Sp[3] = 0.0f;
+        else Sp[0] = 0.0f;
     }
 }
 

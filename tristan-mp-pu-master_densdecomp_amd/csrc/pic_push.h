@@ -155,8 +155,10 @@ __device__ __forceinline__ void gather(const Src& src, bool quirk2, int i, int j
 // address for the particle's 4x4x4 node block, every node at a constant
 // offset from it (LDS reads with immediate offsets, no per-node address
 // arithmetic).  (i,j,k) is the particle's cell in staged-node coordinates,
-// at least 1 on every axis.  Same arithmetic as gather<3, ORD>.
-template <int ORD, int FX, int FY>
+// at least 1 on every axis.  Same arithmetic as gather<3, ORD>.  RB node
+// rows go into each batch of reads (two batches in flight): 2 hides LDS
+// latency within the wave, 1 keeps the registers for more waves.
+template <int ORD, int FX, int FY, int RB = 2>
 __device__ __forceinline__ void gather3_lds(const float* __restrict__ s, bool quirk2, int i, int j, int k, float ox,
                                             float oy, float oz, float* E)
 {
@@ -167,7 +169,6 @@ __device__ __forceinline__ void gather3_lds(const float* __restrict__ s, bool qu
     gather_weights<ORD>(oz, quirk2, Spz, Sdz, diz);
     constexpr int N0 = GatherRange<ORD>::N0, N1 = GatherRange<ORD>::N1, NB = N1 - N0;
     constexpr int NR = NB * NB;          // node rows (y, z) of the block
-    constexpr int RB = 2;                // rows per batch of reads
     constexpr int NBATCH = NR / RB;
     constexpr int BW = RB * NB * 3;      // float2 per batch
 #pragma unroll
