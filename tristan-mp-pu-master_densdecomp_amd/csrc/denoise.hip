@@ -8,11 +8,12 @@
 //         (DIM 3): 3 layers of 3x3x3 conv, width 16.
 // Input: the raw deposited current J / s (3 channels, padded to 4).
 //
-// Convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x16bf16_1k):
+// Convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x32_bf16):
 // activations channels-last in BF16, FP32 accumulation.  The same kernel
 // computes the backward pass w.r.t. the input (flipped, transposed weights);
 // the weight gradient is its own MFMA kernel reducing over pixels.
 // Training (Adam) runs on the device; master weights are FP32.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -72,39 +73,73 @@ __device__ __forceinline__ int64_t neighbour(const Geo& g, int64_t p, int tap)
 
 // out[p][co] = bias[co] + sum_{tap,ci} W[co][tap*cin + ci] * in[nb(p,tap)][ci]
 // in: BF16 [P][cin] (cin multiple of 4); W: BF16 [cout_pad][kpad] with
-// kpad = taps*cin rounded up to 16 (zeros beyond); out: FP32 [P][cout]
+// kpad = taps*cin rounded up to 32 (zeros beyond); out: FP32 [P][cout]
 // (cout <= cout_pad, multiple of 16 for cout_pad).
 // gelu_out (optional): BF16 gelu(out) [P][cout]; raw_out (optional): FP32.
-__device__ __forceinline__ float gelu(float z)
+// GELU, tanh form: 0.5 z (1 + tanh u) with u = k0 (z + k1 z^3), evaluated
+// as z sigmoid(2u) = z / (1 + exp(-2u)) with the hardware exponential (a few
+// instructions instead of a library tanh; its relative error, ~1e-7, is far
+// below the BF16 rounding of the activations).  exp(-2u) overflows to +inf
+// for large negative z, which gives -0, the limit.
+__device__ __forceinline__ float gelu_sig(float z)
 {
     const float k0 = 0.7978845608f, k1 = 0.044715f;
-    return 0.5f * z * (1.0f + tanhf(k0 * (z + k1 * z * z * z)));
+    const float u = k0 * z * (1.0f + k1 * z * z);
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+}
+__device__ __forceinline__ float gelu(float z)
+{
+    return z * gelu_sig(z);
 }
 __device__ __forceinline__ float gelu_grad(float z)
 {
     const float k0 = 0.7978845608f, k1 = 0.044715f;
-    const float u = k0 * (z + k1 * z * z * z);
-    const float th = tanhf(u);
-    return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * k0 * (1.0f + 3.0f * k1 * z * z);
+    const float sg = gelu_sig(z);
+    const float th = 2.0f * sg - 1.0f;    // tanh u
+    return sg + 0.5f * z * (1.0f - th * th) * k0 * (1.0f + 3.0f * k1 * z * z);
 }
 
-// A workgroup computes one tile of TX x TY x TZ pixels (2D 64x4, 3D 16x4x4)
-// for all cout_pad channels.  Its weights (KS K steps of 16) are loaded into
-// registers first, so their latency hides behind the staging of the tile's
-// input with a one-pixel periodic halo in LDS.  The LDS pixel stride is
-// CIN + 4 halfwords for CIN >= 16, so the 16 pixels of an MFMA row group
-// fall in distinct banks.  Each of the 4 waves owns 4 row segments of 16
-// pixels (the MFMA M dimension); a K step of 16 (4 channels x 4 lane
-// groups) reads its A operands from LDS, and its weights serve all 4
-// segments.
-template <int DIM>
+// A persistent workgroup of 8 waves loops over tiles of 512 pixels (3D
+// 16x8x4, 2D 64x8; half as many rows when the layer has 32 output channels)
+// and computes all cout_pad channels of each with the gfx950 MFMA
+// v_mfma_f32_16x16x32_bf16: M = 16 pixels of a row segment, N = 16 output
+// channels, K = 32 of (tap, input channel).  Its weights (KS K steps of 32)
+// stay in registers for all its tiles.  The input of the next tile (with a
+// one-pixel periodic halo) is loaded from memory into registers while the
+// MFMAs of this tile run on the LDS copy of this one; the outputs go through
+// a second LDS region, so the stores are whole 16-byte units of consecutive
+// pixels.  A lane of K group kq supplies 8 consecutive K values: 8 channels
+// of one tap (16 bytes) for 16 or 32 input channels, two taps of 4 channels
+// for 4.  The LDS pixel stride is CIN + 8 halfwords for CIN >= 16 (48 or 80
+// bytes), so each group of 8 lanes of a 16-byte read covers distinct banks.
+// A and B take their K values in the same lane order, so the product does
+// not depend on how the instruction orders K inside a lane group.
+constexpr int kConvThreads = 512;
+template <int DIM, int NT>
 struct ConvTile {
-    static constexpr int TX = DIM == 3 ? 16 : 64, TY = 4, TZ = DIM == 3 ? 4 : 1;
+    static constexpr int TX = DIM == 3 ? 16 : 64, TY = NT == 1 ? 8 : 4, TZ = DIM == 3 ? 4 : 1;
     static constexpr int HX = TX + 2, HY = TY + 2, HZ = DIM == 3 ? TZ + 2 : 1;
-    static constexpr int MPR = TX / 16;   // 16-pixel segments per row
+    static constexpr int MPR = TX / 16;                                      // 16-pixel segments per row
+    static constexpr int NPIX = TX * TY * TZ;
+    static constexpr int SPW = NPIX / 16 / (kConvThreads / 64);              // segments per wave
 };
 
-__host__ __device__ constexpr int conv_pixel_stride(int cin) { return cin + (cin >= 16 ? 4 : 0); }
+__host__ __device__ constexpr int conv_pixel_stride(int cin) { return cin + (cin >= 16 ? 8 : 0); }
+// FP32 words per pixel of the output staging (16 * NT channels + 4: the 64
+// lanes of an MFMA result write fall in distinct banks)
+__host__ __device__ constexpr int conv_out_stride(int nt) { return 16 * nt + 4; }
+// the input region of the LDS, in halfwords (a multiple of 8)
+template <int DIM, int NT, int CIN>
+__host__ __device__ constexpr int conv_in_halves()
+{
+    return (ConvTile<DIM, NT>::HX * ConvTile<DIM, NT>::HY * ConvTile<DIM, NT>::HZ * conv_pixel_stride(CIN) + 7) / 8 * 8;
+}
+template <int DIM, int NT, int CIN>
+__host__ __device__ constexpr size_t conv_lds_bytes()
+{
+    return sizeof(uint16_t) * (size_t)conv_in_halves<DIM, NT, CIN>() +
+           sizeof(float) * (size_t)ConvTile<DIM, NT>::NPIX * conv_out_stride(NT);
+}
 
 __device__ __forceinline__ int wrap_once(int x, int n)
 {
@@ -114,102 +149,220 @@ __device__ __forceinline__ int wrap_once(int x, int n)
     return x;
 }
 
-template <int NT, int DIM, int CIN>   // NT = cout_pad / 16
-__global__ void __launch_bounds__(256) k_conv(Geo g, const uint16_t* __restrict__ in,
-                                              const uint16_t* __restrict__ W,
-                                              const float* __restrict__ bias, int cout,
-                                              float* __restrict__ raw_out, uint16_t* __restrict__ gelu_out,
-                                              int gelu_stride)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ bf16x8 bf16x8_of(uint4 v)
 {
-    using T = ConvTile<DIM>;
-    constexpr int TAPS = DIM == 3 ? 27 : 9, K = TAPS * CIN, KS = (K + 15) / 16, KPAD = KS * 16;
-    constexpr int PS = conv_pixel_stride(CIN), CU = CIN / 4;
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+// BF16 bits of x, round to nearest even (the hardware conversion; f2bf's
+// rounding for finite values)
+__device__ __forceinline__ uint16_t bf16_bits(float x)
+{
+    return __builtin_bit_cast(uint16_t, (__bf16)x);
+}
+
+template <int NT, int DIM, int CIN>   // NT = cout_pad / 16
+__global__ void __launch_bounds__(kConvThreads) k_conv(Geo g, const uint16_t* __restrict__ in,
+                                                       const uint16_t* __restrict__ W,
+                                                       const float* __restrict__ bias, int cout,
+                                                       float* __restrict__ raw_out, uint16_t* __restrict__ gelu_out,
+                                                       int gelu_stride)
+{
+    using T = ConvTile<DIM, NT>;
+    constexpr int TAPS = DIM == 3 ? 27 : 9, K = TAPS * CIN, KS = (K + 31) / 32, KPAD = KS * 32;
+    constexpr int PS = conv_pixel_stride(CIN), SPW = T::SPW, NPIX = T::NPIX, OS = conv_out_stride(NT);
+    constexpr int NTH = kConvThreads;
+    static_assert(CIN == 4 || CIN % 8 == 0, "input channels: 4 or a multiple of 8");
     extern __shared__ uint16_t sm[];
+    uint16_t* si = sm;                                                         // input tile
+    float* so = reinterpret_cast<float*>(sm + conv_in_halves<DIM, NT, CIN>());  // output tile
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int i = lane & 15, kq = lane >> 4;
-    bf16x4 bw[KS][NT];
+    bf16x8 bw[KS][NT];
 #pragma unroll
     for (int s = 0; s < KS; ++s)
 #pragma unroll
-        for (int n = 0; n < NT; ++n) {
-            const uint2 w = *reinterpret_cast<const uint2*>(W + (n * 16 + i) * KPAD + s * 16 + 4 * kq);
-            bw[s][n][0] = (short)(w.x & 0xffff); bw[s][n][1] = (short)(w.x >> 16);
-            bw[s][n][2] = (short)(w.y & 0xffff); bw[s][n][3] = (short)(w.y >> 16);
-        }
+        for (int n = 0; n < NT; ++n)
+            bw[s][n] = bf16x8_of(*reinterpret_cast<const uint4*>(W + (n * 16 + i) * KPAD + s * 32 + 8 * kq));
     const int ntx = (g.nx + T::TX - 1) / T::TX, nty = (g.ny + T::TY - 1) / T::TY;
     const int ntz = (g.nz + T::TZ - 1) / T::TZ;
-    int64_t bid = blockIdx.x;
-    const int tx = (int)(bid % ntx);
-    bid /= ntx;
-    const int ty = (int)(bid % nty);
-    bid /= nty;
-    const int tz = (int)(bid % ntz);
-    const int64_t ibase = (bid / ntz) * g.npix;
-    const int x0 = tx * T::TX, y0 = ty * T::TY, z0 = tz * T::TZ;
-    // stage the haloed tile: units of 4 channels (8 bytes)
-    constexpr int NUNITS = T::HX * T::HY * T::HZ * CU;
-    for (int u = threadIdx.x; u < NUNITS; u += 256) {
-        const int pix = u / CU, c4 = u - pix * CU;
-        const int hx = pix % T::HX, hy = (pix / T::HX) % T::HY, hz = pix / (T::HX * T::HY);
-        const int x = wrap_once(x0 + hx - 1, g.nx), y = wrap_once(y0 + hy - 1, g.ny);
-        const int z = DIM == 3 ? wrap_once(z0 + hz - 1, g.nz) : 0;
-        const int64_t q = ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
-        *reinterpret_cast<uint2*>(sm + pix * PS + 4 * c4) = *reinterpret_cast<const uint2*>(in + q * CIN + 4 * c4);
-    }
-    __syncthreads();
-    int cen[4];   // LDS pixel of this lane's A row in each segment
+    const int64_t ntiles = (int64_t)ntx * nty * ntz * g.batch;
+    struct Org { int x0, y0, z0; int64_t ibase; };
+    auto origin = [&](int64_t t) {
+        Org o;
+        o.x0 = (int)(t % ntx) * T::TX;
+        t /= ntx;
+        o.y0 = (int)(t % nty) * T::TY;
+        t /= nty;
+        o.z0 = (int)(t % ntz) * T::TZ;
+        o.ibase = (t / ntz) * g.npix;
+        return o;
+    };
+    // the haloed input tile in units of 8 channels (16 bytes), or 4 (8 bytes)
+    constexpr int CU = CIN >= 8 ? 8 : 4, NU = CIN / CU;
+    constexpr int NUNITS = T::HX * T::HY * T::HZ * NU, PER = (NUNITS + NTH - 1) / NTH;
+    using Unit = typename std::conditional<CU == 8, u32x4, u32x2>::type;
+    Unit stage[PER];
+    // this thread's units, the same in every tile: halo coordinates packed
+    // as hx | hy << 8 | hz << 16 (-1: none) and the LDS destination
+    int hpos[PER], hdst[PER];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-        const int m = wave * 4 + mt, row = m / T::MPR, lx = (m % T::MPR) * 16 + i;
+    for (int j = 0; j < PER; ++j) {
+        const int u = threadIdx.x + NTH * j;
+        hpos[j] = -1;
+        hdst[j] = 0;
+        if (u < NUNITS) {
+            const int pix = u / NU, cu = u - pix * NU;
+            const int hx = pix % T::HX, hy = (pix / T::HX) % T::HY, hz = pix / (T::HX * T::HY);
+            hpos[j] = hx | (hy << 8) | (hz << 16);
+            hdst[j] = pix * PS + CU * cu;
+        }
+    }
+    auto load_tile = [&](const Org& o) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            if (hpos[j] >= 0) {
+                const int hx = hpos[j] & 0xff, hy = (hpos[j] >> 8) & 0xff, hz = hpos[j] >> 16;
+                const int x = wrap_once(o.x0 + hx - 1, g.nx), y = wrap_once(o.y0 + hy - 1, g.ny);
+                const int z = DIM == 3 ? wrap_once(o.z0 + hz - 1, g.nz) : 0;
+                const int64_t q = o.ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+                // the unit's channels: its LDS offset within the pixel
+                const int cu = (hdst[j] % PS) / CU;
+                stage[j] = *reinterpret_cast<const Unit*>(in + q * CIN + CU * cu);
+            }
+        }
+    };
+    auto store_tile = [&]() {
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (hpos[j] >= 0) *reinterpret_cast<Unit*>(si + hdst[j]) = stage[j];
+    };
+    int cen[SPW];   // LDS pixel of this lane's A row in each segment
+#pragma unroll
+    for (int mt = 0; mt < SPW; ++mt) {
+        const int m = wave * SPW + mt, row = m / T::MPR, lx = (m % T::MPR) * 16 + i;
         const int ly = row % T::TY, lz = row / T::TY;
         cen[mt] = (((DIM == 3 ? lz + 1 : 0) * T::HY + ly + 1) * T::HX + lx + 1) * PS;
     }
-    f32x4 acc[4][NT];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-        for (int n = 0; n < NT; ++n) acc[mt][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const int k = s * 16 + 4 * kq;
-        const bool kv = k < K;
-        const int tap = kv ? k / CIN : 0, ci = k - tap * CIN;
+    auto tap_off = [](int tap) {
         const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = (DIM == 3) ? tap / 9 - 1 : 0;
-        const int doff = ((dz * T::HY + dy) * T::HX + dx) * PS + ci;
+        return ((dz * T::HY + dy) * T::HX + dx) * PS;
+    };
+    int64_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    Org cur = origin(t);
+    load_tile(cur);
+    store_tile();
+    __syncthreads();
+    for (;;) {
+        const int64_t tn = t + gridDim.x;
+        const bool more = tn < ntiles;
+        Org nxt = cur;
+        if (more) {
+            nxt = origin(tn);
+            load_tile(nxt);   // in flight during the MFMAs below
+        }
+        f32x4 acc[SPW][NT];
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            bf16x4 a = bf16x4{0, 0, 0, 0};
-            if (kv) {
-                const uint2 v = *reinterpret_cast<const uint2*>(sm + cen[mt] + doff);
-                a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16);
-                a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
+        for (int mt = 0; mt < SPW; ++mt)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[mt][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = s * 32 + 8 * kq;    // this lane's first K value
+            // LDS offsets of this lane's K values (out-of-range K reads tap 0
+            // and is masked to zero)
+            int off0, off1 = 0;
+            bool v0, v1 = false;
+            if constexpr (CIN >= 8) {
+                v0 = k < K;
+                const int tap = v0 ? k / CIN : 0;
+                off0 = tap_off(tap) + (v0 ? k - tap * CIN : 0);
+            } else {
+                const int t0 = k / 4, t1 = t0 + 1;
+                v0 = t0 < TAPS;
+                v1 = t1 < TAPS;
+                off0 = tap_off(v0 ? t0 : 0);
+                off1 = tap_off(v1 ? t1 : 0);
             }
+#pragma unroll
+            for (int mt = 0; mt < SPW; ++mt) {
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if constexpr (CIN >= 8) {
+                    if (v0) v = *reinterpret_cast<const uint4*>(si + cen[mt] + off0);
+                } else {
+                    if (v0) { const uint2 h = *reinterpret_cast<const uint2*>(si + cen[mt] + off0); v.x = h.x; v.y = h.y; }
+                    if (v1) { const uint2 h = *reinterpret_cast<const uint2*>(si + cen[mt] + off1); v.z = h.x; v.w = h.y; }
+                }
+                const bf16x8 a = bf16x8_of(v);
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[s][n], acc[mt][n], 0, 0, 0);
+            }
+        }
+        // all waves are done with the input tile and with the previous
+        // tile's outputs: D[i][j] (j = lane % 16 the channel, i = 4*(lane/16)
+        // + r the pixel) goes to the output region, the next input to the
+        // input region
+        __syncthreads();
+#pragma unroll
+        for (int mt = 0; mt < SPW; ++mt) {
+            const int m = wave * SPW + mt;
 #pragma unroll
             for (int n = 0; n < NT; ++n)
-                acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[s][n], acc[mt][n], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) so[(m * 16 + 4 * kq + r) * OS + n * 16 + i] = acc[mt][n][r];
         }
-    }
-    // D[i][j]: j = lane % 16 (channel), i = 4*(lane/16) + r (pixel)
+        if (more) store_tile();
+        __syncthreads();
+        // pixel e of the tile (x fastest) -> its global index, or -1 outside the grid
+        auto gpix = [&](int e) -> int64_t {
+            const int x = cur.x0 + e % T::TX, y = cur.y0 + (e / T::TX) % T::TY, z = cur.z0 + e / (T::TX * T::TY);
+            if (x >= g.nx || y >= g.ny || z >= g.nz) return -1;
+            return cur.ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+        };
+        if (gelu_out) {
+            if (cout % 8 == 0 && gelu_stride % 8 == 0) {
+                // 8 channels (16 bytes) per thread and unit; consecutive
+                // threads take consecutive pixels of one channel group, so
+                // their LDS reads (stride OS words) fall in distinct banks
+                const int nu = cout / 8;
+                for (int u = threadIdx.x; u < NPIX * nu; u += NTH) {
+                    const int cu = u / NPIX, e = u - cu * NPIX, c0 = 8 * cu;
+                    const int64_t pp = gpix(e);
+                    if (pp < 0) continue;
+                    uint32_t w[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-        const int m = wave * 4 + mt, row = m / T::MPR;
-        const int y = y0 + row % T::TY, z = z0 + row / T::TY;
-        if (y >= g.ny || z >= g.nz) continue;
-#pragma unroll
-        for (int n = 0; n < NT; ++n) {
-            const int co = n * 16 + (lane & 15);
-            if (co >= cout) continue;
-            const float bv = bias ? bias[co] : 0.0f;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int x = x0 + (m % T::MPR) * 16 + 4 * kq + r;
-                if (x >= g.nx) continue;
-                const int64_t pp = ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
-                const float zv = acc[mt][n][r] + bv;
-                if (raw_out) raw_out[pp * cout + co] = zv;
-                if (gelu_out) gelu_out[pp * gelu_stride + co] = f2bf(gelu(zv));
+                    for (int h = 0; h < 4; ++h) {
+                        const float z0v = so[e * OS + c0 + 2 * h] + (bias ? bias[c0 + 2 * h] : 0.0f);
+                        const float z1v = so[e * OS + c0 + 2 * h + 1] + (bias ? bias[c0 + 2 * h + 1] : 0.0f);
+                        w[h] = (uint32_t)bf16_bits(gelu(z0v)) | ((uint32_t)bf16_bits(gelu(z1v)) << 16);
+                    }
+                    *reinterpret_cast<uint4*>(gelu_out + pp * gelu_stride + c0) = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+            } else {
+                for (int u = threadIdx.x; u < NPIX * cout; u += NTH) {
+                    const int e = u / cout, co = u - e * cout;
+                    const int64_t pp = gpix(e);
+                    if (pp >= 0) gelu_out[pp * gelu_stride + co] = bf16_bits(gelu(so[e * OS + co] + (bias ? bias[co] : 0.0f)));
+                }
             }
         }
+        if (raw_out) {
+            // one element per thread: consecutive threads, consecutive addresses
+            for (int u = threadIdx.x; u < NPIX * cout; u += NTH) {
+                const int e = u / cout, co = u - e * cout;
+                const int64_t pp = gpix(e);
+                if (pp >= 0) raw_out[pp * cout + co] = so[e * OS + co] + (bias ? bias[co] : 0.0f);
+            }
+        }
+        if (!more) break;
+        t = tn;
+        cur = nxt;
     }
 }
 
@@ -505,7 +658,7 @@ using namespace tmpg::dn;
 struct DLayer {
     int cin, cout;          // real channel counts (cin includes the input pad to 4)
     int cout_pad;           // multiple of 16 (MFMA N tiles)
-    int taps, kpad;         // K = taps*cin rounded to 16
+    int taps, kpad;         // K = taps*cin rounded to 32
     int cinT, kpadT, cinT_pad;   // transposed conv for the input gradient
     float* w = nullptr;     // FP32 master [cout][kpad]
     float* b = nullptr;     // [cout]
@@ -516,6 +669,7 @@ struct DLayer {
 
 struct tmpg_model {
     int dim, width, nlayers;
+    int ncu = 256;          // compute units of the model's device
     std::vector<DLayer> L;
     float scale = 0.0f;
     int base_passes = 0;    // binomial passes applied to J before the correction
@@ -617,11 +771,17 @@ int launch_conv(tmpg_model* m, const Geo& g, const uint16_t* in, int cin, int ta
 {
     auto go = [&](auto dim, auto nt, auto ci) {
         constexpr int D = decltype(dim)::value, NT = decltype(nt)::value, CIN = decltype(ci)::value;
-        using T = ConvTile<D>;
+        using T = ConvTile<D, NT>;
         const int64_t tiles = (int64_t)cdiv(g.nx, T::TX) * cdiv(g.ny, T::TY) * cdiv(g.nz, T::TZ) * g.batch;
-        const size_t lds = sizeof(uint16_t) * (size_t)T::HX * T::HY * T::HZ * conv_pixel_stride(CIN);
-        hipLaunchKernelGGL((k_conv<NT, D, CIN>), dim3((unsigned)tiles), dim3(256), lds, m->stream, g, in, W, bias,
-                           cout, raw, gout, gstride);
+        constexpr size_t lds = conv_lds_bytes<D, NT, CIN>();
+        static_assert(lds <= 160 * 1024, "conv LDS");
+        // persistent workgroups: as many as fit on the device at once
+        const int per_cu = (int)std::max<size_t>(1, 160 * 1024 / lds);
+        const int64_t grid = std::min<int64_t>(tiles, (int64_t)m->ncu * per_cu);
+        (void)hipFuncSetAttribute((const void*)k_conv<NT, D, CIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL((k_conv<NT, D, CIN>), dim3((unsigned)grid), dim3(kConvThreads), lds, m->stream, g, in, W,
+                           bias, cout, raw, gout, gstride);
     };
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
@@ -641,7 +801,7 @@ int launch_conv(tmpg_model* m, const Geo& g, const uint16_t* in, int cin, int ta
     };
     const int width = g.dim == 3 ? 16 : 32;
     const int taps_dim = g.dim == 3 ? 27 : 9;
-    if ((cin != 4 && cin != width) || taps != taps_dim || kpad != (taps * cin + 15) / 16 * 16 ||
+    if ((cin != 4 && cin != width) || taps != taps_dim || kpad != (taps * cin + 31) / 32 * 32 ||
         (cout_pad != 16 && cout_pad != 32)) {
         // unreachable for the two networks built here
         m->err = "no conv kernel for cin " + std::to_string(cin) + ", cout_pad " + std::to_string(cout_pad);
@@ -788,6 +948,12 @@ int tmpg_model_create(int dim, uint64_t seed, tmpg_model** out)
     m->nlayers = (dim == 2) ? 4 : 3;
     const int taps = (dim == 2) ? 9 : 27;
     if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { delete m; return TMPG_EHIP; }
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+            m->ncu = ncu;
+    }
     uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
     auto rnd = [&]() {  // uniform (-1,1), LCG + xorshift
         st = st * 6364136223846793005ull + 1442695040888963407ull;
@@ -801,10 +967,10 @@ int tmpg_model_create(int dim, uint64_t seed, tmpg_model** out)
         L.cout = (l == m->nlayers - 1) ? 3 : m->width;
         L.cout_pad = (L.cout + 15) / 16 * 16;
         L.taps = taps;
-        L.kpad = (taps * L.cin + 15) / 16 * 16;
+        L.kpad = (taps * L.cin + 31) / 32 * 32;
         L.cinT = (L.cout + 3) / 4 * 4;
         L.cinT_pad = (L.cin + 15) / 16 * 16;
-        L.kpadT = (taps * L.cinT + 15) / 16 * 16;
+        L.kpadT = (taps * L.cinT + 31) / 32 * 32;
         const int64_t nw = (int64_t)L.cout * L.kpad;
         rc |= dn_alloc(m, &L.w, nw); rc |= dn_alloc(m, &L.b, L.cout);
         rc |= dn_alloc(m, &L.gw, nw); rc |= dn_alloc(m, &L.gb, L.cout);
