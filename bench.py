@@ -143,7 +143,7 @@ def host_cpu():
     return model, max(1, min(n, 16))
 
 
-def cpu_reference_baseline(w, steps=30):
+def cpu_reference_baseline(w, steps=30, max_cores=0):
     """The reference's own Fortran routines (oracle/_ref, built from its
     sources at -O3, FP32) on the host cores: one single-rank process per core,
     each on its own 64x32x32 sample of the workload's plasma (disjoint seeds),
@@ -155,6 +155,8 @@ def cpu_reference_baseline(w, steps=30):
     if not os.path.exists(exe):
         return None
     model, ncore = host_cpu()
+    if max_cores > 0:
+        ncore = min(ncore, max_cores)
     sx, sy, sz = (64, 32, 32) if dim == 3 else (256, 256, 1)
     nm = normalisation(w)
     ng = 5 if order == 1 else 7
@@ -342,6 +344,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ratio", action="store_true", help="skip the denoised / plain step-time ratio")
     ap.add_argument("--no-counters", action="store_true", help="skip the profiled pass for VALU / LDS counters")
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="host cores of the CPU baseline (0: all, at most 16)")
     ap.add_argument("--sort-every", type=int, default=0,
                     help="steps between in-tile cell sorts of the particles (0: library default, <0: never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -405,7 +409,7 @@ def main():
     if not a.no_cpu:
         try:
             print("bench: CPU baseline", file=sys.stderr, flush=True)
-            cpu = cpu_reference_baseline(w)
+            cpu = cpu_reference_baseline(w, max_cores=a.cpu_cores)
         except Exception as e:  # the baseline is reported, never required
             print(f"cpu baseline failed: {e}", file=sys.stderr)
     out = {
