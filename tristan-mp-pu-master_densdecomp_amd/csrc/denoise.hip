@@ -366,6 +366,171 @@ __global__ void __launch_bounds__(kConvThreads) k_conv(Geo g, const uint16_t* __
     }
 }
 
+// The 3D convolution with 16 input and 16 output channels (the hidden and
+// last layers of the 3D network), register-blocked over output rows.  A
+// workgroup of 4 waves computes a 16x8x4-pixel tile; wave w owns the output
+// rows y = 2w, 2w+1 of all 4 z planes (8 row segments of 16 pixels).  An
+// A operand (16 pixels of one input row, K = 32: the 16 channels at two x
+// shifts, dx = -1 and 0, or dx = +1 and zeros) is read from LDS once and
+// used by every output row of the wave within one row of it in y and z,
+// with the weights of that (dy, dz): 6 LDS reads per output row instead of
+// 14, for 18 MFMAs instead of 14 (the zero half of the dx = +1 operand).
+// The 18 weight fragments (9 (dy, dz) x 2 halves) stay in registers.  The
+// output goes through the same LDS region after the MFMAs, for 16-byte
+// stores of consecutive pixels.
+struct RbTile {
+    static constexpr int TX = 16, TY = 8, TZ = 4, HX = 18, HY = 10, HZ = 6, NPIX = 512;
+    static constexpr int PS = 24;   // halfwords per staged pixel (16 channels + 8)
+    static constexpr int OS = 20;   // FP32 words per output pixel
+};
+constexpr size_t conv_rb_lds_bytes()
+{
+    return std::max(sizeof(uint16_t) * (size_t)RbTile::HX * RbTile::HY * RbTile::HZ * RbTile::PS,
+                    sizeof(float) * (size_t)RbTile::NPIX * RbTile::OS);
+}
+
+__global__ void __launch_bounds__(256) k_conv3_rb(Geo g, const uint16_t* __restrict__ in,
+                                                  const uint16_t* __restrict__ W, const float* __restrict__ bias,
+                                                  int cout, float* __restrict__ raw_out,
+                                                  uint16_t* __restrict__ gelu_out, int gelu_stride)
+{
+    using T = RbTile;
+    constexpr int CIN = 16, KPAD = (27 * CIN + 31) / 32 * 32, PS = T::PS, OS = T::OS, NPIX = T::NPIX;
+    extern __shared__ uint16_t sm[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, kq = lane >> 4;
+    // weights: fragment [dy+1][dz+1][h]; lane group kq < 2 holds dx = -1
+    // (h = 0) or +1 (h = 1), kq >= 2 dx = 0 (h = 0) or zeros (h = 1);
+    // channels 8 * (kq & 1) .. +7; tap = (dz+1)*9 + (dy+1)*3 + (dx+1)
+    bf16x8 bw[3][3][2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                const int dxs = kq < 2 ? (h == 0 ? 0 : 2) : (h == 0 ? 1 : -1);   // dx + 1, -1: zeros
+                if (dxs >= 0) {
+                    const int tap = c * 9 + a * 3 + dxs;
+                    v = *reinterpret_cast<const uint4*>(W + i * KPAD + tap * CIN + 8 * (kq & 1));
+                }
+                bw[a][c][h] = bf16x8_of(v);
+            }
+    const int ntx = (g.nx + T::TX - 1) / T::TX, nty = (g.ny + T::TY - 1) / T::TY;
+    const int ntz = (g.nz + T::TZ - 1) / T::TZ;
+    int64_t bid = blockIdx.x;
+    const int x0 = (int)(bid % ntx) * T::TX;
+    bid /= ntx;
+    const int y0 = (int)(bid % nty) * T::TY;
+    bid /= nty;
+    const int z0 = (int)(bid % ntz) * T::TZ;
+    const int64_t ibase = (bid / ntz) * g.npix;
+    // stage the haloed tile, 16-byte units (8 channels): every thread issues
+    // all its loads before its first LDS store (one memory latency per tile)
+    constexpr int NUNITS = T::HX * T::HY * T::HZ * 2, PER = (NUNITS + 255) / 256, SB = PER;
+#pragma unroll
+    for (int j0 = 0; j0 < PER; j0 += SB) {
+        u32x4 v[SB];
+#pragma unroll
+        for (int j = 0; j < SB; ++j) {
+            const int u = threadIdx.x + 256 * (j0 + j);
+            if (j0 + j < PER && u < NUNITS) {
+                const int pix = u >> 1, cu = u & 1;
+                const int hx = pix % T::HX, hy = (pix / T::HX) % T::HY, hz = pix / (T::HX * T::HY);
+                const int x = wrap_once(x0 + hx - 1, g.nx), y = wrap_once(y0 + hy - 1, g.ny);
+                const int z = wrap_once(z0 + hz - 1, g.nz);
+                const int64_t q = ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+                v[j] = *reinterpret_cast<const u32x4*>(in + q * CIN + 8 * cu);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < SB; ++j) {
+            const int u = threadIdx.x + 256 * (j0 + j);
+            if (j0 + j < PER && u < NUNITS)
+                *reinterpret_cast<u32x4*>(sm + (u >> 1) * PS + 8 * (u & 1)) = v[j];
+        }
+    }
+    __syncthreads();
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // this lane's A column: pixel i of the row, shifted by the half's dx
+    const int dxl0 = kq < 2 ? -1 : 0, dxl1 = kq < 2 ? 1 : 0;
+    const bool live1 = kq < 2;
+    const int ci = 8 * (kq & 1);
+#pragma unroll
+    for (int zi = 0; zi < 6; ++zi)          // input plane z = zi - 1 (halo coordinate zi)
+#pragma unroll
+        for (int ry = 0; ry < 4; ++ry) {    // input row y = 2 wave + ry - 1 (halo coordinate 2 wave + ry)
+            const int rowoff = ((zi * T::HY) + 2 * wave + ry) * T::HX;
+            const uint4 v0 = *reinterpret_cast<const uint4*>(sm + (rowoff + i + 1 + dxl0) * PS + ci);
+            uint4 v1 = make_uint4(0u, 0u, 0u, 0u);
+            if (live1) v1 = *reinterpret_cast<const uint4*>(sm + (rowoff + i + 1 + dxl1) * PS + ci);
+            const bf16x8 a0 = bf16x8_of(v0), a1 = bf16x8_of(v1);
+#pragma unroll
+            for (int yl = 0; yl < 2; ++yl)
+#pragma unroll
+                for (int zo = 0; zo < 4; ++zo) {
+                    const int dy = ry - 1 - yl, dz = zi - 1 - zo;
+                    if (dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;
+                    acc[yl][zo] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw[dy + 1][dz + 1][0], acc[yl][zo], 0, 0, 0);
+                    acc[yl][zo] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw[dy + 1][dz + 1][1], acc[yl][zo], 0, 0, 0);
+                }
+        }
+    // outputs through LDS (the input region, free after this barrier):
+    // D[i][j] with j = lane % 16 the channel, i = 4*(lane/16) + r the pixel
+    __syncthreads();
+    float* so = reinterpret_cast<float*>(sm);
+#pragma unroll
+    for (int yl = 0; yl < 2; ++yl)
+#pragma unroll
+        for (int zo = 0; zo < 4; ++zo) {
+            const int row = zo * T::TY + 2 * wave + yl;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) so[(row * 16 + 4 * kq + r) * OS + i] = acc[yl][zo][r];
+        }
+    __syncthreads();
+    auto gpix = [&](int e) -> int64_t {
+        const int x = x0 + e % T::TX, y = y0 + (e / T::TX) % T::TY, z = z0 + e / (T::TX * T::TY);
+        if (x >= g.nx || y >= g.ny || z >= g.nz) return -1;
+        return ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+    };
+    if (gelu_out) {
+        if (cout % 8 == 0 && gelu_stride % 8 == 0) {
+            const int nu = cout / 8;
+            for (int u = threadIdx.x; u < NPIX * nu; u += 256) {
+                const int cu = u / NPIX, e = u - cu * NPIX, c0 = 8 * cu;
+                const int64_t pp = gpix(e);
+                if (pp < 0) continue;
+                uint32_t w[4];
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const float z0v = so[e * OS + c0 + 2 * h] + (bias ? bias[c0 + 2 * h] : 0.0f);
+                    const float z1v = so[e * OS + c0 + 2 * h + 1] + (bias ? bias[c0 + 2 * h + 1] : 0.0f);
+                    w[h] = (uint32_t)bf16_bits(gelu(z0v)) | ((uint32_t)bf16_bits(gelu(z1v)) << 16);
+                }
+                *reinterpret_cast<uint4*>(gelu_out + pp * gelu_stride + c0) = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+        } else {
+            for (int u = threadIdx.x; u < NPIX * cout; u += 256) {
+                const int e = u / cout, co = u - e * cout;
+                const int64_t pp = gpix(e);
+                if (pp >= 0) gelu_out[pp * gelu_stride + co] = bf16_bits(gelu(so[e * OS + co] + (bias ? bias[co] : 0.0f)));
+            }
+        }
+    }
+    if (raw_out) {
+        for (int u = threadIdx.x; u < NPIX * cout; u += 256) {
+            const int e = u / cout, co = u - e * cout;
+            const int64_t pp = gpix(e);
+            if (pp >= 0) raw_out[pp * cout + co] = so[e * OS + co] + (bias ? bias[co] : 0.0f);
+        }
+    }
+}
+
 // dW[co][k] += sum_p dz[p][co] * patch(in)[p][k], db[co] += sum_p dz[p][co].
 // MFMA with M = co, N = k, reduction over 16 pixels per instruction; a block
 // sums a chunk of pixels for all output tiles and adds to dW atomically.
@@ -771,6 +936,17 @@ int launch_conv(tmpg_model* m, const Geo& g, const uint16_t* in, int cin, int ta
 {
     auto go = [&](auto dim, auto nt, auto ci) {
         constexpr int D = decltype(dim)::value, NT = decltype(nt)::value, CIN = decltype(ci)::value;
+        if constexpr (D == 3 && NT == 1 && CIN == 16) {
+            // the register-blocked kernel: one tile per workgroup, three
+            // workgroups per CU
+            const int64_t tiles = (int64_t)cdiv(g.nx, RbTile::TX) * cdiv(g.ny, RbTile::TY) *
+                                  cdiv(g.nz, RbTile::TZ) * g.batch;
+            constexpr size_t lds = conv_rb_lds_bytes();
+            (void)hipFuncSetAttribute((const void*)k_conv3_rb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(k_conv3_rb, dim3((unsigned)tiles), dim3(256), lds, m->stream, g, in, W, bias, cout,
+                               raw, gout, gstride);
+            return;
+        }
         using T = ConvTile<D, NT>;
         const int64_t tiles = (int64_t)cdiv(g.nx, T::TX) * cdiv(g.ny, T::TY) * cdiv(g.nz, T::TZ) * g.batch;
         constexpr size_t lds = conv_lds_bytes<D, NT, CIN>();
