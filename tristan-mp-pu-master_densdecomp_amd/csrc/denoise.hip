@@ -375,7 +375,9 @@ __global__ void __launch_bounds__(kConvThreads) k_conv(Geo g, const uint16_t* __
 // used by every output row of the wave within one row of it in y and z,
 // with the weights of that (dy, dz): 6 LDS reads per output row instead of
 // 14, for 18 MFMAs instead of 14 (the zero half of the dx = +1 operand).
-// The 18 weight fragments (9 (dy, dz) x 2 halves) stay in registers.  The
+// The workgroups are persistent: the 18 weight fragments (9 (dy, dz) x 2
+// halves) are loaded into registers once, and the next tile's input is
+// loaded into registers while this tile's MFMAs and stores run.  The
 // output goes through the same LDS region after the MFMAs, for 16-byte
 // stores of consecutive pixels.
 struct RbTile {
@@ -389,7 +391,8 @@ constexpr size_t conv_rb_lds_bytes()
                     sizeof(float) * (size_t)RbTile::NPIX * RbTile::OS);
 }
 
-__global__ void __launch_bounds__(256) k_conv3_rb(Geo g, const uint16_t* __restrict__ in,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_conv3_rb(Geo g, const uint16_t* __restrict__ in,
                                                   const uint16_t* __restrict__ W, const float* __restrict__ bias,
                                                   int cout, float* __restrict__ raw_out,
                                                   uint16_t* __restrict__ gelu_out, int gelu_stride)
@@ -419,115 +422,156 @@ __global__ void __launch_bounds__(256) k_conv3_rb(Geo g, const uint16_t* __restr
             }
     const int ntx = (g.nx + T::TX - 1) / T::TX, nty = (g.ny + T::TY - 1) / T::TY;
     const int ntz = (g.nz + T::TZ - 1) / T::TZ;
-    int64_t bid = blockIdx.x;
-    const int x0 = (int)(bid % ntx) * T::TX;
-    bid /= ntx;
-    const int y0 = (int)(bid % nty) * T::TY;
-    bid /= nty;
-    const int z0 = (int)(bid % ntz) * T::TZ;
-    const int64_t ibase = (bid / ntz) * g.npix;
-    // stage the haloed tile, 16-byte units (8 channels): every thread issues
-    // all its loads before its first LDS store (one memory latency per tile)
-    constexpr int NUNITS = T::HX * T::HY * T::HZ * 2, PER = (NUNITS + 255) / 256, SB = PER;
+    const int64_t ntiles = (int64_t)ntx * nty * ntz * g.batch;
+    struct Org { int x0, y0, z0; int64_t ibase; };
+    auto origin = [&](int64_t t) {
+        Org o;
+        o.x0 = (int)(t % ntx) * T::TX;
+        t /= ntx;
+        o.y0 = (int)(t % nty) * T::TY;
+        t /= nty;
+        o.z0 = (int)(t % ntz) * T::TZ;
+        o.ibase = (t / ntz) * g.npix;
+        return o;
+    };
+    // the haloed input tile in 16-byte units (8 channels); a thread's units
+    // are the same in every tile: halo coordinates packed hx | hy<<8 | hz<<16
+    constexpr int NUNITS = T::HX * T::HY * T::HZ * 2, PER = (NUNITS + 255) / 256;
+    u32x4 stage[PER];
+    int hpos[PER];
 #pragma unroll
-    for (int j0 = 0; j0 < PER; j0 += SB) {
-        u32x4 v[SB];
-#pragma unroll
-        for (int j = 0; j < SB; ++j) {
-            const int u = threadIdx.x + 256 * (j0 + j);
-            if (j0 + j < PER && u < NUNITS) {
-                const int pix = u >> 1, cu = u & 1;
-                const int hx = pix % T::HX, hy = (pix / T::HX) % T::HY, hz = pix / (T::HX * T::HY);
-                const int x = wrap_once(x0 + hx - 1, g.nx), y = wrap_once(y0 + hy - 1, g.ny);
-                const int z = wrap_once(z0 + hz - 1, g.nz);
-                const int64_t q = ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
-                v[j] = *reinterpret_cast<const u32x4*>(in + q * CIN + 8 * cu);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < SB; ++j) {
-            const int u = threadIdx.x + 256 * (j0 + j);
-            if (j0 + j < PER && u < NUNITS)
-                *reinterpret_cast<u32x4*>(sm + (u >> 1) * PS + 8 * (u & 1)) = v[j];
+    for (int j = 0; j < PER; ++j) {
+        const int u = threadIdx.x + 256 * j;
+        hpos[j] = -1;
+        if (u < NUNITS) {
+            const int pix = u >> 1;
+            hpos[j] = (pix % T::HX) | (((pix / T::HX) % T::HY) << 8) | ((pix / (T::HX * T::HY)) << 16);
         }
     }
-    __syncthreads();
-    f32x4 acc[2][4];
+    auto load_tile = [&](const Org& o) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int j = 0; j < PER; ++j) {
+            if (hpos[j] >= 0) {
+                const int u = threadIdx.x + 256 * j;
+                const int hx = hpos[j] & 0xff, hy = (hpos[j] >> 8) & 0xff, hz = hpos[j] >> 16;
+                const int x = wrap_once(o.x0 + hx - 1, g.nx), y = wrap_once(o.y0 + hy - 1, g.ny);
+                const int z = wrap_once(o.z0 + hz - 1, g.nz);
+                const int64_t q = o.ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+                stage[j] = *reinterpret_cast<const u32x4*>(in + q * CIN + 8 * (u & 1));
+            }
+        }
+    };
+    auto store_tile = [&]() {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < PER; ++j) {
+            const int u = threadIdx.x + 256 * j;
+            if (hpos[j] >= 0) *reinterpret_cast<u32x4*>(sm + (u >> 1) * PS + 8 * (u & 1)) = stage[j];
+        }
+    };
     // this lane's A column: pixel i of the row, shifted by the half's dx
     const int dxl0 = kq < 2 ? -1 : 0, dxl1 = kq < 2 ? 1 : 0;
     const bool live1 = kq < 2;
     const int ci = 8 * (kq & 1);
-#pragma unroll
-    for (int zi = 0; zi < 6; ++zi)          // input plane z = zi - 1 (halo coordinate zi)
-#pragma unroll
-        for (int ry = 0; ry < 4; ++ry) {    // input row y = 2 wave + ry - 1 (halo coordinate 2 wave + ry)
-            const int rowoff = ((zi * T::HY) + 2 * wave + ry) * T::HX;
-            const uint4 v0 = *reinterpret_cast<const uint4*>(sm + (rowoff + i + 1 + dxl0) * PS + ci);
-            uint4 v1 = make_uint4(0u, 0u, 0u, 0u);
-            if (live1) v1 = *reinterpret_cast<const uint4*>(sm + (rowoff + i + 1 + dxl1) * PS + ci);
-            const bf16x8 a0 = bf16x8_of(v0), a1 = bf16x8_of(v1);
-#pragma unroll
-            for (int yl = 0; yl < 2; ++yl)
-#pragma unroll
-                for (int zo = 0; zo < 4; ++zo) {
-                    const int dy = ry - 1 - yl, dz = zi - 1 - zo;
-                    if (dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;
-                    acc[yl][zo] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw[dy + 1][dz + 1][0], acc[yl][zo], 0, 0, 0);
-                    acc[yl][zo] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw[dy + 1][dz + 1][1], acc[yl][zo], 0, 0, 0);
-                }
-        }
-    // outputs through LDS (the input region, free after this barrier):
-    // D[i][j] with j = lane % 16 the channel, i = 4*(lane/16) + r the pixel
-    __syncthreads();
     float* so = reinterpret_cast<float*>(sm);
-#pragma unroll
-    for (int yl = 0; yl < 2; ++yl)
-#pragma unroll
-        for (int zo = 0; zo < 4; ++zo) {
-            const int row = zo * T::TY + 2 * wave + yl;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) so[(row * 16 + 4 * kq + r) * OS + i] = acc[yl][zo][r];
-        }
+    // persistent: tiles blockIdx.x, + gridDim.x, ...; the next tile's input
+    // is loaded into registers while this tile's MFMAs and stores run, and
+    // the one LDS region holds the input, then the output, of each tile
+    int64_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    Org cur = origin(t);
+    load_tile(cur);
+    store_tile();
     __syncthreads();
-    auto gpix = [&](int e) -> int64_t {
-        const int x = x0 + e % T::TX, y = y0 + (e / T::TX) % T::TY, z = z0 + e / (T::TX * T::TY);
-        if (x >= g.nx || y >= g.ny || z >= g.nz) return -1;
-        return ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
-    };
-    if (gelu_out) {
-        if (cout % 8 == 0 && gelu_stride % 8 == 0) {
-            const int nu = cout / 8;
-            for (int u = threadIdx.x; u < NPIX * nu; u += 256) {
-                const int cu = u / NPIX, e = u - cu * NPIX, c0 = 8 * cu;
-                const int64_t pp = gpix(e);
-                if (pp < 0) continue;
-                uint32_t w[4];
+    for (;;) {
+        const int64_t tn = t + gridDim.x;
+        const bool more = tn < ntiles;
+        Org nxt = cur;
+        if (more) {
+            nxt = origin(tn);
+            load_tile(nxt);
+        }
+        f32x4 acc[2][4];
 #pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    const float z0v = so[e * OS + c0 + 2 * h] + (bias ? bias[c0 + 2 * h] : 0.0f);
-                    const float z1v = so[e * OS + c0 + 2 * h + 1] + (bias ? bias[c0 + 2 * h + 1] : 0.0f);
-                    w[h] = (uint32_t)bf16_bits(gelu(z0v)) | ((uint32_t)bf16_bits(gelu(z1v)) << 16);
-                }
-                *reinterpret_cast<uint4*>(gelu_out + pp * gelu_stride + c0) = make_uint4(w[0], w[1], w[2], w[3]);
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[a][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int zi = 0; zi < 6; ++zi)          // input plane z = zi - 1 (halo coordinate zi)
+#pragma unroll
+            for (int ry = 0; ry < 4; ++ry) {    // input row y = 2 wave + ry - 1 (halo coordinate 2 wave + ry)
+                const int rowoff = ((zi * T::HY) + 2 * wave + ry) * T::HX;
+                const uint4 v0 = *reinterpret_cast<const uint4*>(sm + (rowoff + i + 1 + dxl0) * PS + ci);
+                uint4 v1 = make_uint4(0u, 0u, 0u, 0u);
+                if (live1) v1 = *reinterpret_cast<const uint4*>(sm + (rowoff + i + 1 + dxl1) * PS + ci);
+                const bf16x8 a0 = bf16x8_of(v0), a1 = bf16x8_of(v1);
+#pragma unroll
+                for (int yl = 0; yl < 2; ++yl)
+#pragma unroll
+                    for (int zo = 0; zo < 4; ++zo) {
+                        const int dy = ry - 1 - yl, dz = zi - 1 - zo;
+                        if (dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;
+                        acc[yl][zo] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw[dy + 1][dz + 1][0], acc[yl][zo], 0, 0, 0);
+                        acc[yl][zo] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw[dy + 1][dz + 1][1], acc[yl][zo], 0, 0, 0);
+                    }
             }
-        } else {
+        // outputs through the LDS region (the input is no longer read):
+        // D[i][j] with j = lane % 16 the channel, i = 4*(lane/16) + r the pixel
+        __syncthreads();
+#pragma unroll
+        for (int yl = 0; yl < 2; ++yl)
+#pragma unroll
+            for (int zo = 0; zo < 4; ++zo) {
+                const int row = zo * T::TY + 2 * wave + yl;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) so[(row * 16 + 4 * kq + r) * OS + i] = acc[yl][zo][r];
+            }
+        __syncthreads();
+        auto gpix = [&](int e) -> int64_t {
+            const int x = cur.x0 + e % T::TX, y = cur.y0 + (e / T::TX) % T::TY, z = cur.z0 + e / (T::TX * T::TY);
+            if (x >= g.nx || y >= g.ny || z >= g.nz) return -1;
+            return cur.ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+        };
+        if (gelu_out) {
+            if (cout % 8 == 0 && gelu_stride % 8 == 0) {
+                const int nu = cout / 8;
+                for (int u = threadIdx.x; u < NPIX * nu; u += 256) {
+                    const int cu = u / NPIX, e = u - cu * NPIX, c0 = 8 * cu;
+                    const int64_t pp = gpix(e);
+                    if (pp < 0) continue;
+                    uint32_t w[4];
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const float z0v = so[e * OS + c0 + 2 * h] + (bias ? bias[c0 + 2 * h] : 0.0f);
+                        const float z1v = so[e * OS + c0 + 2 * h + 1] + (bias ? bias[c0 + 2 * h + 1] : 0.0f);
+                        w[h] = (uint32_t)bf16_bits(gelu(z0v)) | ((uint32_t)bf16_bits(gelu(z1v)) << 16);
+                    }
+                    *reinterpret_cast<uint4*>(gelu_out + pp * gelu_stride + c0) = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+            } else {
+                for (int u = threadIdx.x; u < NPIX * cout; u += 256) {
+                    const int e = u / cout, co = u - e * cout;
+                    const int64_t pp = gpix(e);
+                    if (pp >= 0)
+                        gelu_out[pp * gelu_stride + co] = bf16_bits(gelu(so[e * OS + co] + (bias ? bias[co] : 0.0f)));
+                }
+            }
+        }
+        if (raw_out) {
             for (int u = threadIdx.x; u < NPIX * cout; u += 256) {
                 const int e = u / cout, co = u - e * cout;
                 const int64_t pp = gpix(e);
-                if (pp >= 0) gelu_out[pp * gelu_stride + co] = bf16_bits(gelu(so[e * OS + co] + (bias ? bias[co] : 0.0f)));
+                if (pp >= 0) raw_out[pp * cout + co] = so[e * OS + co] + (bias ? bias[co] : 0.0f);
             }
         }
-    }
-    if (raw_out) {
-        for (int u = threadIdx.x; u < NPIX * cout; u += 256) {
-            const int e = u / cout, co = u - e * cout;
-            const int64_t pp = gpix(e);
-            if (pp >= 0) raw_out[pp * cout + co] = so[e * OS + co] + (bias ? bias[co] : 0.0f);
-        }
+        if (!more) break;
+        // the stores above read the region; then it takes the next input
+        __syncthreads();
+        store_tile();
+        __syncthreads();
+        t = tn;
+        cur = nxt;
     }
 }
 
@@ -937,13 +981,18 @@ int launch_conv(tmpg_model* m, const Geo& g, const uint16_t* in, int cin, int ta
     auto go = [&](auto dim, auto nt, auto ci) {
         constexpr int D = decltype(dim)::value, NT = decltype(nt)::value, CIN = decltype(ci)::value;
         if constexpr (D == 3 && NT == 1 && CIN == 16) {
-            // the register-blocked kernel: one tile per workgroup, three
-            // workgroups per CU
+            // the register-blocked kernel: persistent workgroups, as many as
+            // fit on the device (three per CU by LDS)
             const int64_t tiles = (int64_t)cdiv(g.nx, RbTile::TX) * cdiv(g.ny, RbTile::TY) *
                                   cdiv(g.nz, RbTile::TZ) * g.batch;
             constexpr size_t lds = conv_rb_lds_bytes();
             (void)hipFuncSetAttribute((const void*)k_conv3_rb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipLaunchKernelGGL(k_conv3_rb, dim3((unsigned)tiles), dim3(256), lds, m->stream, g, in, W, bias, cout,
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3_rb, 256, lds) != hipSuccess ||
+                per_cu < 1)
+                per_cu = 1;
+            const int64_t grid = std::min<int64_t>(tiles, (int64_t)m->ncu * per_cu);
+            hipLaunchKernelGGL(k_conv3_rb, dim3((unsigned)grid), dim3(256), lds, m->stream, g, in, W, bias, cout,
                                raw, gout, gstride);
             return;
         }
@@ -951,11 +1000,15 @@ int launch_conv(tmpg_model* m, const Geo& g, const uint16_t* in, int cin, int ta
         const int64_t tiles = (int64_t)cdiv(g.nx, T::TX) * cdiv(g.ny, T::TY) * cdiv(g.nz, T::TZ) * g.batch;
         constexpr size_t lds = conv_lds_bytes<D, NT, CIN>();
         static_assert(lds <= 160 * 1024, "conv LDS");
-        // persistent workgroups: as many as fit on the device at once
-        const int per_cu = (int)std::max<size_t>(1, 160 * 1024 / lds);
-        const int64_t grid = std::min<int64_t>(tiles, (int64_t)m->ncu * per_cu);
         (void)hipFuncSetAttribute((const void*)k_conv<NT, D, CIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
+        // persistent workgroups: as many as are resident on the device at once
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv<NT, D, CIN>, kConvThreads, lds) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        const int64_t grid = std::min<int64_t>(tiles, (int64_t)m->ncu * per_cu);
         hipLaunchKernelGGL((k_conv<NT, D, CIN>), dim3((unsigned)grid), dim3(kConvThreads), lds, m->stream, g, in, W,
                            bias, cout, raw, gout, gstride);
     };
