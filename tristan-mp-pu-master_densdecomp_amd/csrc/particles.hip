@@ -302,6 +302,7 @@ __host__ __device__ inline CellsLds cells_lds(int FX, int FY, int FZ, int JX, in
     L.nc = nc;
     size_t o = 0;
     L.j_off = o;   o += sizeof(double) * 3 * (size_t)L.jplane;
+    o = (o + 15) & ~(size_t)15;    // the staged fields are read 16 bytes at a time
     L.f_off = o;   o += sizeof(float) * 6 * (size_t)L.fplane;
     L.rec_off = o; o += sizeof(float) * 6 * (size_t)CELLS_R;
     L.rq_off = o;  o += sizeof(float) * (size_t)CELLS_R;
@@ -543,7 +544,8 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     constexpr int fplane = FX * FY * FZ, jplane = JX * JY * JZ, nc = TX * TY * TZ;
     extern __shared__ __align__(16) unsigned char smb[];
     double* sj = reinterpret_cast<double*>(smb + L.j_off);
-    float* sf = reinterpret_cast<float*>(smb + L.f_off);
+    float4* sf4 = reinterpret_cast<float4*>(smb + L.f_off);
+    float2* sf2 = reinterpret_cast<float2*>(smb + L.f_off + sizeof(float4) * (size_t)fplane);
     float* rec = reinterpret_cast<float*>(smb + L.rec_off);    // [6][R]: ox oy oz dx dy dz
     float* rq = reinterpret_cast<float*>(smb + L.rq_off);
     int* cnt = reinterpret_cast<int*>(smb + L.cnt_off);
@@ -576,10 +578,9 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         const int gj = wrap1(j0 - 1 + lj, a.ny);
         const int gk = wrap1(k0 - 1 + lk, a.nz);
         const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
-        float2* d = reinterpret_cast<float2*>(sf) + 3 * e;   // node-major
-        d[0] = make_float2(a.f6[0][g], a.f6[1][g]);
-        d[1] = make_float2(a.f6[2][g], a.f6[3][g]);
-        d[2] = make_float2(a.f6[4][g], a.f6[5][g]);
+        // two node-major arrays: (Ex, Ey, Ez, Bx) in 16 bytes, (By, Bz) in 8
+        sf4[e] = make_float4(a.f6[0][g], a.f6[1][g], a.f6[2][g], a.f6[3][g]);
+        sf2[e] = make_float2(a.f6[4][g], a.f6[5][g]);
     }
     for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
     for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
@@ -690,7 +691,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             ox = p.ox; oy = p.oy; oz = p.oz;
             {
                 float E[6];
-                gather3_lds<ORD, FX, FY, 1>(sf, (a.flags & PUSH_GATHER2) != 0, li + 1, lj + 1, lk + 1, p.ox, p.oy,
+                gather3_lds<ORD, FX, FY, 1>(sf4, sf2, (a.flags & PUSH_GATHER2) != 0, li + 1, lj + 1, lk + 1, p.ox, p.oy,
                                          p.oz, E);
                 push_fields<VAY>(E, a.flags, a.c, a.cinv, sp.qm, p, dx, dy, dz);
             }

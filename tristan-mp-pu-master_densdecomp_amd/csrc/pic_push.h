@@ -151,16 +151,19 @@ __device__ __forceinline__ void gather(const Src& src, bool quirk2, int i, int j
 }
 
 // The 3D gather from node-major primal fields in LDS whose strides FX, FY
-// (nodes per row, rows per plane) are compile-time constants: one base
-// address for the particle's 4x4x4 node block, every node at a constant
-// offset from it (LDS reads with immediate offsets, no per-node address
-// arithmetic).  (i,j,k) is the particle's cell in staged-node coordinates,
-// at least 1 on every axis.  Same arithmetic as gather<3, ORD>.  RB node
-// rows go into each batch of reads (two batches in flight): 2 hides LDS
-// latency within the wave, 1 keeps the registers for more waves.
+// (nodes per row, rows per plane) are compile-time constants.  A node's six
+// values sit in two arrays, (Ex, Ey, Ez, Bx) 16 bytes in s4 and (By, Bz) 8
+// bytes in s2, so each node is one 16-byte and one 8-byte read; both take a
+// 16-bit immediate offset, which covers the particle's whole 4x4x4 node block
+// (at most 6.4 KB from its first node for the 8x8x4 tile), so the block
+// needs one base address per array and no per-node address arithmetic.
+// (i,j,k) is the particle's cell in staged-node coordinates, at least 1 on
+// every axis.  Same arithmetic as gather<3, ORD>.  RB node rows go into each
+// batch of reads (two batches in flight): 2 hides LDS latency within the
+// wave, 1 keeps the registers for more waves.
 template <int ORD, int FX, int FY, int RB = 2>
-__device__ __forceinline__ void gather3_lds(const float* __restrict__ s, bool quirk2, int i, int j, int k, float ox,
-                                            float oy, float oz, float* E)
+__device__ __forceinline__ void gather3_lds(const float4* __restrict__ s4, const float2* __restrict__ s2, bool quirk2,
+                                            int i, int j, int k, float ox, float oy, float oz, float* E)
 {
     float Spx[4], Sdx[4], Spy[4], Sdy[4], Spz[4], Sdz[4];
     int dix, diy, diz;
@@ -170,31 +173,36 @@ __device__ __forceinline__ void gather3_lds(const float* __restrict__ s, bool qu
     constexpr int N0 = GatherRange<ORD>::N0, N1 = GatherRange<ORD>::N1, NB = N1 - N0;
     constexpr int NR = NB * NB;          // node rows (y, z) of the block
     constexpr int NBATCH = NR / RB;
-    constexpr int BW = RB * NB * 3;      // float2 per batch
+    constexpr int BW = RB * NB;          // nodes per batch
 #pragma unroll
     for (int q = 0; q < 6; ++q) E[q] = 0.0f;
-    const float2* p = reinterpret_cast<const float2*>(s) + 3 * ((i - 1) + FX * ((j - 1) + FY * (k - 1)));
+    const int n0 = (i - 1) + FX * ((j - 1) + FY * (k - 1));
+    const float4* p4 = s4 + n0;
+    const float2* p2 = s2 + n0;
     // the reads of batch n + 1 are issued before batch n is summed, so a
     // wave waits for LDS once per batch instead of once per node
-    float2 buf[2][BW];
-    auto load = [&](int n, float2* d) {
+    float4 b4[2][BW];
+    float2 b2[2][BW];
+    auto load = [&](int n, float4* d4, float2* d2) {
 #pragma unroll
         for (int rr = 0; rr < RB; ++rr) {
             const int r = n * RB + rr;
             const int b = N0 + r % NB, c = N0 + r / NB;
-            const float2* row = p + 3 * FX * (b + FY * c);
+            const int row = FX * (b + FY * c) + N0;
 #pragma unroll
-            for (int t = 0; t < NB; ++t)
-#pragma unroll
-                for (int m = 0; m < 3; ++m) d[(rr * NB + t) * 3 + m] = row[3 * (N0 + t) + m];
+            for (int t = 0; t < NB; ++t) {
+                d4[rr * NB + t] = p4[row + t];
+                d2[rr * NB + t] = p2[row + t];
+            }
         }
     };
-    load(0, buf[0]);
+    load(0, b4[0], b2[0]);
 #pragma unroll
     for (int n = 0; n < NBATCH; ++n) {
-        if (n + 1 < NBATCH) load(n + 1, buf[(n + 1) & 1]);
+        if (n + 1 < NBATCH) load(n + 1, b4[(n + 1) & 1], b2[(n + 1) & 1]);
         __builtin_amdgcn_sched_barrier(0);
-        const float2* d = buf[n & 1];
+        const float4* d4 = b4[n & 1];
+        const float2* d2 = b2[n & 1];
 #pragma unroll
         for (int rr = 0; rr < RB; ++rr) {
             const int r = n * RB + rr;
@@ -203,13 +211,14 @@ __device__ __forceinline__ void gather3_lds(const float* __restrict__ s, bool qu
 #pragma unroll
             for (int t = 0; t < NB; ++t) {
                 const float w = Spx[N0 + t];
-                const float2 a0 = d[(rr * NB + t) * 3], a1 = d[(rr * NB + t) * 3 + 1], a2 = d[(rr * NB + t) * 3 + 2];
-                acc[0] += a0.x * w;
-                acc[1] += a0.y * w;
-                acc[2] += a1.x * w;
-                acc[3] += a1.y * w;
-                acc[4] += a2.x * w;
-                acc[5] += a2.y * w;
+                const float4 a = d4[rr * NB + t];
+                const float2 m = d2[rr * NB + t];
+                acc[0] += a.x * w;
+                acc[1] += a.y * w;
+                acc[2] += a.z * w;
+                acc[3] += a.w * w;
+                acc[4] += m.x * w;
+                acc[5] += m.y * w;
             }
             const float wyz = Spy[b] * Spz[c];
 #pragma unroll
