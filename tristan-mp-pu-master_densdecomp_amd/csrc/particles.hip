@@ -484,17 +484,20 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
 #pragma unroll
             for (int u = 0; u < 3; ++u) { run += dSy[u]; Cy[u] = q * run; }
         }
+        // every update is a chain of fused multiply-adds into the sum (the
+        // compiler does not reassociate "s += a*b + c*d" into two of them)
+        const float dzt = twelfth * dzc;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const float wyz = Ay[b] * azc + twelfth * dSy[b] * dzc;
+            const float wyz = fmaf(Ay[b], azc, dSy[b] * dzt);
 #pragma unroll
-            for (int u = 0; u < 5; ++u) ax[u][b] += Cx[u] * wyz;
+            for (int u = 0; u < 5; ++u) ax[u][b] = fmaf(Cx[u], wyz, ax[u][b]);
         }
 #pragma unroll
         for (int u = 0; u < 6; ++u) {
-            const float wxz = Ax[u] * azc + twelfth * dSx[u] * dzc;
+            const float wxz = fmaf(Ax[u], azc, dSx[u] * dzt);
 #pragma unroll
-            for (int b = 0; b < 3; ++b) ay[u][b] += Cy[b] * wxz;
+            for (int b = 0; b < 3; ++b) ay[u][b] = fmaf(Cy[b], wxz, ay[u][b]);
         }
         if (has_z) {
             // Jz: czc (Ax Ay + dSx dSy / 12) as two rank-one updates
@@ -503,7 +506,7 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
             for (int u = 0; u < 6; ++u) {
                 const float ga = czc * Ax[u], gd = czt * dSx[u];
 #pragma unroll
-                for (int b = 0; b < 4; ++b) az[u][b] += ga * Ay[b] + gd * dSy[b];
+                for (int b = 0; b < 4; ++b) az[u][b] = fmaf(gd, dSy[b], fmaf(ga, Ay[b], az[u][b]));
             }
         }
     }

@@ -109,23 +109,29 @@ __device__ __forceinline__ void gather(const Src& src, bool quirk2, int i, int j
         int cx[4];
 #pragma unroll
         for (int t = N0; t < N1; ++t) cx[t] = src.col(i - 1 + t);
+        // each z plane summed with the x-y weights, then scaled by its z
+        // weight (the order of every sum is the one of gather3_lds)
+        float wxy[4][4];
+#pragma unroll
+        for (int b = N0; b < N1; ++b)
+#pragma unroll
+            for (int t = N0; t < N1; ++t) wxy[b][t] = Spx[t] * Spy[b];
 #pragma unroll
         for (int c = N0; c < N1; ++c) {
+            float acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int b = N0; b < N1; ++b) {
                 const auto r = src.row(j - 1 + b, k - 1 + c);
-                float acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
                 for (int t = N0; t < N1; ++t) {
                     float v[6];
                     src.get6(r, cx[t], v);
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) acc[q] += v[q] * Spx[t];
+                    for (int q = 0; q < 6; ++q) acc[q] = fmaf(v[q], wxy[b][t], acc[q]);
                 }
-                const float wyz = Spy[b] * Spz[c];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) E[q] += acc[q] * wyz;
             }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) E[q] = fmaf(acc[q], Spz[c], E[q]);
         }
     } else {
         int cp[4], cd[4];
@@ -197,6 +203,14 @@ __device__ __forceinline__ void gather3_lds(const float4* __restrict__ s4, const
         }
     };
     load(0, b4[0], b2[0]);
+    // a z plane of the block is summed with the x-y weights, then scaled by
+    // its z weight: NB*NB products plus 6 (NB^3 + NB) multiply-adds
+    float wxy[NB][NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int t = 0; t < NB; ++t) wxy[b][t] = Spx[N0 + t] * Spy[N0 + b];
+    float acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int n = 0; n < NBATCH; ++n) {
         if (n + 1 < NBATCH) load(n + 1, b4[(n + 1) & 1], b2[(n + 1) & 1]);
@@ -206,23 +220,27 @@ __device__ __forceinline__ void gather3_lds(const float4* __restrict__ s4, const
 #pragma unroll
         for (int rr = 0; rr < RB; ++rr) {
             const int r = n * RB + rr;
-            const int b = N0 + r % NB, c = N0 + r / NB;
-            float acc[6] = {0, 0, 0, 0, 0, 0};
+            const int b = r % NB, c = r / NB;
 #pragma unroll
             for (int t = 0; t < NB; ++t) {
-                const float w = Spx[N0 + t];
+                const float w = wxy[b][t];
                 const float4 a = d4[rr * NB + t];
                 const float2 m = d2[rr * NB + t];
-                acc[0] += a.x * w;
-                acc[1] += a.y * w;
-                acc[2] += a.z * w;
-                acc[3] += a.w * w;
-                acc[4] += m.x * w;
-                acc[5] += m.y * w;
+                acc[0] = fmaf(a.x, w, acc[0]);
+                acc[1] = fmaf(a.y, w, acc[1]);
+                acc[2] = fmaf(a.z, w, acc[2]);
+                acc[3] = fmaf(a.w, w, acc[3]);
+                acc[4] = fmaf(m.x, w, acc[4]);
+                acc[5] = fmaf(m.y, w, acc[5]);
             }
-            const float wyz = Spy[b] * Spz[c];
+            if (b == NB - 1) {
+                const float wz = Spz[N0 + c];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) E[q] += acc[q] * wyz;
+                for (int q = 0; q < 6; ++q) {
+                    E[q] = fmaf(acc[q], wz, E[q]);
+                    acc[q] = 0.0f;
+                }
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -447,17 +465,28 @@ __device__ __forceinline__ void deposit3_plane(const Sink& snk, int i, int j, in
         }
     }
     const float twelfth = 1.0f / 12.0f;
+    const float dszt = twelfth * dsz;
+    // the plane's weights, each a fused multiply-add: wyz(b) = Ay az + dSy
+    // dsz / 12, wxz(t) = Ax az + dSx dsz / 12; Jz as cz Ax Ay + (cz / 12) dSx dSy
+    float wxz[W], ga[W], gd[W];
+    const float czt = twelfth * cz;
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+        wxz[t] = fmaf(Ax[t], az, dSx[t] * dszt);
+        ga[t] = cz * Ax[t];
+        gd[t] = czt * dSx[t];
+    }
     int cc[W];
 #pragma unroll
     for (int t = 0; t < W; ++t) cc[t] = snk.col(i + bx + t);
 #pragma unroll
     for (int b = 0; b < W; ++b) {
         const auto r = snk.row(j + by + b, k + bz + zc);
-        const float wyz = Ay[b] * az + twelfth * dSy[b] * dsz;
+        const float wyz = fmaf(Ay[b], az, dSy[b] * dszt);
 #pragma unroll
         for (int t = 0; t < W; ++t) {
             if (t < W - 1) snk.add(0, r, cc[t], Cx[t] * wyz);
-            if (b < W - 1) snk.add(1, r, cc[t], Cy[b] * (Ax[t] * az + twelfth * dSx[t] * dsz));
+            if (b < W - 1) snk.add(1, r, cc[t], Cy[b] * wxz[t]);
         }
     }
     // the last plane's z prefix is the telescoped zero
@@ -466,7 +495,7 @@ __device__ __forceinline__ void deposit3_plane(const Sink& snk, int i, int j, in
         for (int b = 0; b < W; ++b) {
             const auto r = snk.row(j + by + b, k + bz + zc);
 #pragma unroll
-            for (int t = 0; t < W; ++t) snk.add(2, r, cc[t], cz * (Ax[t] * Ay[b] + twelfth * dSx[t] * dSy[b]));
+            for (int t = 0; t < W; ++t) snk.add(2, r, cc[t], fmaf(gd[t], dSy[b], ga[t] * Ay[b]));
         }
     }
 }
