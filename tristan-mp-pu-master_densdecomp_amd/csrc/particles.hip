@@ -303,7 +303,7 @@ __host__ __device__ inline CellsLds cells_lds(int FX, int FY, int FZ, int JX, in
     size_t o = 0;
     L.j_off = o;   o += sizeof(double) * 3 * (size_t)L.jplane;
     o = (o + 15) & ~(size_t)15;    // the staged fields are read 16 bytes at a time
-    L.f_off = o;   o += sizeof(float) * 6 * (size_t)L.fplane;
+    L.f_off = o;   o += sizeof(float) * 8 * (size_t)L.fplane;
     L.rec_off = o; o += sizeof(float) * 6 * (size_t)CELLS_R;
     L.rq_off = o;  o += sizeof(float) * (size_t)CELLS_R;
     L.cnt_off = o; o += sizeof(int) * (size_t)nc;
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     extern __shared__ __align__(16) unsigned char smb[];
     double* sj = reinterpret_cast<double*>(smb + L.j_off);
     float4* sf4 = reinterpret_cast<float4*>(smb + L.f_off);
-    float2* sf2 = reinterpret_cast<float2*>(smb + L.f_off + sizeof(float4) * (size_t)fplane);
+    float4* sfp = reinterpret_cast<float4*>(smb + L.f_off + sizeof(float4) * (size_t)fplane);
     float* rec = reinterpret_cast<float*>(smb + L.rec_off);    // [6][R]: ox oy oz dx dy dz
     float* rq = reinterpret_cast<float*>(smb + L.rq_off);
     int* cnt = reinterpret_cast<int*>(smb + L.cnt_off);
@@ -581,9 +581,13 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         const int gj = wrap1(j0 - 1 + lj, a.ny);
         const int gk = wrap1(k0 - 1 + lk, a.nz);
         const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
-        // two node-major arrays: (Ex, Ey, Ez, Bx) in 16 bytes, (By, Bz) in 8
+        // two node-major arrays of 16-byte entries: (Ex, Ey, Ez, Bx) of the
+        // node, and (By, Bz) of the node and of its +x neighbour
         sf4[e] = make_float4(a.f6[0][g], a.f6[1][g], a.f6[2][g], a.f6[3][g]);
-        sf2[e] = make_float2(a.f6[4][g], a.f6[5][g]);
+        const float2 byz = make_float2(a.f6[4][g], a.f6[5][g]);
+        float2* half = reinterpret_cast<float2*>(sfp);
+        half[2 * e] = byz;
+        if (e > 0) half[2 * e - 1] = byz;
     }
     for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
     for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
@@ -694,7 +698,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             ox = p.ox; oy = p.oy; oz = p.oz;
             {
                 float E[6];
-                gather3_lds<ORD, FX, FY, 1>(sf4, sf2, (a.flags & PUSH_GATHER2) != 0, li + 1, lj + 1, lk + 1, p.ox, p.oy,
+                gather3_lds<ORD, FX, FY, 1>(sf4, sfp, (a.flags & PUSH_GATHER2) != 0, li + 1, lj + 1, lk + 1, p.ox, p.oy,
                                          p.oz, E);
                 push_fields<VAY>(E, a.flags, a.c, a.cinv, sp.qm, p, dx, dy, dz);
             }

@@ -158,17 +158,18 @@ __device__ __forceinline__ void gather(const Src& src, bool quirk2, int i, int j
 
 // The 3D gather from node-major primal fields in LDS whose strides FX, FY
 // (nodes per row, rows per plane) are compile-time constants.  A node's six
-// values sit in two arrays, (Ex, Ey, Ez, Bx) 16 bytes in s4 and (By, Bz) 8
-// bytes in s2, so each node is one 16-byte and one 8-byte read; both take a
-// 16-bit immediate offset, which covers the particle's whole 4x4x4 node block
-// (at most 6.4 KB from its first node for the 8x8x4 tile), so the block
-// needs one base address per array and no per-node address arithmetic.
+// values sit in two arrays of 16-byte entries: (Ex, Ey, Ez, Bx) in s4, and
+// in sp (By, Bz) of the node followed by (By, Bz) of its +x neighbour.  A row
+// of the block is NB reads of s4 and NB/2 of sp, all 16-byte reads with a
+// 16-bit immediate offset, which covers the particle's whole 4x4x4 node
+// block (at most 6.4 KB from its first node for the 8x8x4 tile): one base
+// address per array and no per-node address arithmetic.
 // (i,j,k) is the particle's cell in staged-node coordinates, at least 1 on
 // every axis.  Same arithmetic as gather<3, ORD>.  RB node rows go into each
 // batch of reads (two batches in flight): 2 hides LDS latency within the
 // wave, 1 keeps the registers for more waves.
 template <int ORD, int FX, int FY, int RB = 2>
-__device__ __forceinline__ void gather3_lds(const float4* __restrict__ s4, const float2* __restrict__ s2, bool quirk2,
+__device__ __forceinline__ void gather3_lds(const float4* __restrict__ s4, const float4* __restrict__ sp, bool quirk2,
                                             int i, int j, int k, float ox, float oy, float oz, float* E)
 {
     float Spx[4], Sdx[4], Spy[4], Sdy[4], Spz[4], Sdz[4];
@@ -183,23 +184,23 @@ __device__ __forceinline__ void gather3_lds(const float4* __restrict__ s4, const
 #pragma unroll
     for (int q = 0; q < 6; ++q) E[q] = 0.0f;
     const int n0 = (i - 1) + FX * ((j - 1) + FY * (k - 1));
+    static_assert(NB % 2 == 0, "node pairs");
     const float4* p4 = s4 + n0;
-    const float2* p2 = s2 + n0;
+    const float4* pp = sp + n0;
     // the reads of batch n + 1 are issued before batch n is summed, so a
     // wave waits for LDS once per batch instead of once per node
     float4 b4[2][BW];
-    float2 b2[2][BW];
-    auto load = [&](int n, float4* d4, float2* d2) {
+    float4 b2[2][BW / 2];
+    auto load = [&](int n, float4* d4, float4* d2) {
 #pragma unroll
         for (int rr = 0; rr < RB; ++rr) {
             const int r = n * RB + rr;
             const int b = N0 + r % NB, c = N0 + r / NB;
             const int row = FX * (b + FY * c) + N0;
 #pragma unroll
-            for (int t = 0; t < NB; ++t) {
-                d4[rr * NB + t] = p4[row + t];
-                d2[rr * NB + t] = p2[row + t];
-            }
+            for (int t = 0; t < NB; ++t) d4[rr * NB + t] = p4[row + t];
+#pragma unroll
+            for (int t = 0; t < NB; t += 2) d2[(rr * NB + t) / 2] = pp[row + t];
         }
     };
     load(0, b4[0], b2[0]);
@@ -216,7 +217,7 @@ __device__ __forceinline__ void gather3_lds(const float4* __restrict__ s4, const
         if (n + 1 < NBATCH) load(n + 1, b4[(n + 1) & 1], b2[(n + 1) & 1]);
         __builtin_amdgcn_sched_barrier(0);
         const float4* d4 = b4[n & 1];
-        const float2* d2 = b2[n & 1];
+        const float4* d2 = b2[n & 1];
 #pragma unroll
         for (int rr = 0; rr < RB; ++rr) {
             const int r = n * RB + rr;
@@ -225,7 +226,8 @@ __device__ __forceinline__ void gather3_lds(const float4* __restrict__ s4, const
             for (int t = 0; t < NB; ++t) {
                 const float w = wxy[b][t];
                 const float4 a = d4[rr * NB + t];
-                const float2 m = d2[rr * NB + t];
+                const float4 pr = d2[(rr * NB + t) / 2];
+                const float2 m = (t & 1) ? make_float2(pr.z, pr.w) : make_float2(pr.x, pr.y);
                 acc[0] = fmaf(a.x, w, acc[0]);
                 acc[1] = fmaf(a.y, w, acc[1]);
                 acc[2] = fmaf(a.z, w, acc[2]);
