@@ -598,9 +598,11 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     const LdsSink snk{sj, JX, JY, jplane};
     const int lane = __lane_id();
     const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    // tile of a cell: the tile extents are powers of two in the default tile
-    auto tile_at = [&](int i, int j, int k) -> int64_t {
-        return (int64_t)(i / TX) + (int64_t)tl.ntx * ((j / TY) + (int64_t)tl.nty * (k / TZ));
+    // tile of a (wrapped, so non-negative) cell: the tile extents are powers
+    // of two in the default tile, and a grid of fewer than 2^31 cells has
+    // fewer than 2^23 such tiles, so 32-bit arithmetic holds the index
+    auto tile_at = [&](int i, int j, int k) -> int {
+        return (int)((unsigned)i / TX + (unsigned)tl.ntx * ((unsigned)j / TY + (unsigned)tl.nty * ((unsigned)k / TZ)));
     };
 
     // ---- the records so far: crossers deposited directly, the rest
@@ -689,7 +691,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         Part p = pn;
         an = p0 + NT + tid < n;
         if (an) load_part<3, HAS_CH, HAS_ID>(sp.a, base + p0 + NT + tid, a.pk, pn);
-        int64_t td = -1;
+        int td = -1;
         float ox = 0.0f, oy = 0.0f, oz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f, q = 0.0f;
         int li = 0, lj = 0, lk = 0;
         bool fits = false;
@@ -711,7 +713,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         // wave's stayers one block in this tile.  Both global atomics are
         // issued here and their results used only after the deposit record
         // is written, so the record work covers their latency.
-        const bool stay = active && td == t;
+        const bool stay = active && (int64_t)td == t;
         const uint64_t m = __ballot(stay);
         int32_t lslot = 0, b0 = 0;
         if (active && !stay) lslot = atomicAdd(sp.cnt_b + td, 1);

@@ -311,19 +311,15 @@ __device__ __forceinline__ void vay(float c, float cinv, float ex0, float ey0, f
 // offset in [0,1)
 __device__ __forceinline__ int advance_offset(float& o, float dl)
 {
-    float t = o + dl;
-    if (t >= 1.0f) {
-        float r = dl - (1.0f - o);
-        o = r < 0.0f ? 0.0f : r;
-        return 1;
-    }
-    if (t < 0.0f) {
-        float r = 1.0f - ((-dl) - o);
-        o = r >= 1.0f ? 0.99999994f : r;
-        return -1;
-    }
-    o = t;
-    return 0;
+    // all three cases evaluated and selected (no divergent branches)
+    const float t = o + dl;
+    const bool up = t >= 1.0f, down = t < 0.0f;
+    const float ru = dl - (1.0f - o);
+    const float rd = 1.0f - ((-dl) - o);
+    const float vu = ru < 0.0f ? 0.0f : ru;
+    const float vd = rd >= 1.0f ? 0.99999994f : rd;
+    o = up ? vu : (down ? vd : t);
+    return (int)up - (int)down;
 }
 
 }  // namespace tmpg
