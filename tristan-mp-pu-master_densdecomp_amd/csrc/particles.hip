@@ -409,14 +409,14 @@ __device__ __forceinline__ float quad_bcast(float v)
 }
 
 // Per-cell register sums of one cell's records (idx[r0 .. r0+n) index the
-// records) at z window slot Q (0..3) of the cell: Jx and Jy at z node slot Q,
-// Jz at prefix slot Q (slots 0..2; Q = 3 has no Jz), added to the tile's LDS
-// currents once.  The four slots of a cell are four adjacent lanes (a quad)
-// that walk the cell's records together: lane Q computes the shape profile
-// of axis Q (x, y, z; lane 3 repeats x) and the quad shares the three
-// profiles by DPP broadcasts, so each record's shapes are computed once.
-// A lane holds 62 sums, which keeps the kernel in 128 VGPRs (four waves per
-// SIMD).  The y and z profiles come from the 6-node window, whose nodes
+// records) for slot Q (0..3) of the cell: Jx and Jy at z node slot Q, and Jz
+// at y node slot Q of all three z prefix slots (so the Jz work is spread over
+// the four lanes), added to the tile's LDS currents once.  The four slots of
+// a cell are four adjacent lanes (a quad) that walk the cell's records
+// together: lane Q computes the shape profile of axis Q (x, y, z; lane 3
+// repeats x) and the quad shares the three profiles by DPP broadcasts, so
+// each record's shapes are computed once.  A lane holds 56 sums, which keeps
+// the kernel in 128 VGPRs (four waves per SIMD).  The y and z profiles come from the 6-node window, whose nodes
 // -1..2 are the 4-node window of a record that keeps its y/z piece.
 template <int ORD>
 __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, int ty,
@@ -427,9 +427,8 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
     const int r1 = r0 + n;
     if (r1 == r0) return;
     const int li = c % tx, lj = (c / tx) % ty, lk = c / (tx * ty);
-    const bool has_z = Q < 3;
     const int axis = Q == 3 ? 0 : Q;
-    float ax[5][4], ay[6][3], az[6][4];
+    float ax[5][4], ay[6][3], az[3][6];
 #pragma unroll
     for (int u = 0; u < 5; ++u)
 #pragma unroll
@@ -439,9 +438,9 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
 #pragma unroll
         for (int b = 0; b < 3; ++b) ay[u][b] = 0.0f;
 #pragma unroll
-    for (int u = 0; u < 6; ++u)
+    for (int s = 0; s < 3; ++s)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) az[u][b] = 0.0f;
+        for (int u = 0; u < 6; ++u) az[s][u] = 0.0f;
     for (int rr = r0; rr < r1; ++rr) {
         const int r = idx[rr];
         const float o = rec[axis * CELLS_R + r], d = rec[(3 + axis) * CELLS_R + r];
@@ -461,7 +460,7 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
             dSy[b] = quad_bcast<1>(dS6[1 + b]);
             Ay[b] = sy + 0.5f * dSy[b];
         }
-        float azc, dzc, czc;
+        float azc, dzc, cz0, cz1, cz2;
         {
             float Sz[4], dSz[4];
 #pragma unroll
@@ -473,7 +472,9 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
             const float S = Q == 0 ? Sz[0] : Q == 1 ? Sz[1] : Q == 2 ? Sz[2] : Sz[3];
             dzc = Q == 0 ? dSz[0] : Q == 1 ? dSz[1] : Q == 2 ? dSz[2] : dSz[3];
             azc = S + 0.5f * dzc;
-            czc = q * (Q == 0 ? s0 : Q == 1 ? s1 : s2);
+            cz0 = q * s0;
+            cz1 = q * s1;
+            cz2 = q * s2;
         }
         float Cx[5], Cy[3];
         {
@@ -499,14 +500,17 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
 #pragma unroll
             for (int b = 0; b < 3; ++b) ay[u][b] = fmaf(Cy[b], wxz, ay[u][b]);
         }
-        if (has_z) {
-            // Jz: czc (Ax Ay + dSx dSy / 12) as two rank-one updates
-            const float czt = twelfth * czc;
+        {
+            // Jz at y slot Q: cz(s) (Ax Ay(Q) + dSx dSy(Q) / 12) for the three
+            // z prefix slots s
+            const float ayq = Q == 0 ? Ay[0] : Q == 1 ? Ay[1] : Q == 2 ? Ay[2] : Ay[3];
+            const float dyq = twelfth * (Q == 0 ? dSy[0] : Q == 1 ? dSy[1] : Q == 2 ? dSy[2] : dSy[3]);
 #pragma unroll
             for (int u = 0; u < 6; ++u) {
-                const float ga = czc * Ax[u], gd = czt * dSx[u];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) az[u][b] = fmaf(gd, dSy[b], fmaf(ga, Ay[b], az[u][b]));
+                const float w = fmaf(Ax[u], ayq, dSx[u] * dyq);
+                az[0][u] = fmaf(cz0, w, az[0][u]);
+                az[1][u] = fmaf(cz1, w, az[1][u]);
+                az[2][u] = fmaf(cz2, w, az[2][u]);
             }
         }
     }
@@ -524,13 +528,11 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
 #pragma unroll
         for (int u = 0; u < 6; ++u) snk.add(1, row, li + u, ay[u][b]);
     }
-    if (has_z) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int row = snk.row(lj + 1 + b, lk + 1 + Q);
+    for (int s = 0; s < 3; ++s) {
+        const int row = snk.row(lj + 1 + Q, lk + 1 + s);
 #pragma unroll
-            for (int u = 0; u < 6; ++u) snk.add(2, row, li + u, az[u][b]);
-        }
+        for (int u = 0; u < 6; ++u) snk.add(2, row, li + u, az[s][u]);
     }
 }
 
