@@ -564,42 +564,49 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     // (stored from 0 up), high 16 bits y/z crossers (stored from the top down)
     unsigned* nrec = reinterpret_cast<unsigned*>(smb + L.bytes - 16);
 
-    const int nb = gridDim.x;
-    int64_t t = blockIdx.x;
-    if (a.remap) t = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
     const Tiling& tl = a.tl;
-    const int ti = (int)(t % tl.ntx);
-    const int tj = (int)((t / tl.ntx) % tl.nty);
-    const int tk = (int)(t / ((int64_t)tl.ntx * tl.nty));
-    const int i0 = ti * TX, j0 = tj * TY, k0 = tk * TZ;
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
     constexpr int NT = CELLS_T;
-    for (int e = tid; e < fplane; e += NT) {
-        const int li = e % FX;
-        const int r = e / FX;
+    static_assert(fplane <= NT, "one staged node per thread");
+    // persistent workgroups: workgroup b takes the tiles tbase, tbase +
+    // tstride, ... below tend.  With a.remap (the grid and the tile count
+    // multiples of 8) the workgroups of one XCD (b mod 8) share one
+    // contiguous eighth of the tiles, so neighbouring tiles' halos meet in
+    // that XCD's L2.
+    const int G = gridDim.x;
+    int64_t tbase = blockIdx.x, tstride = G, tend = tl.ntiles;
+    if (a.remap) {
+        const int64_t per = tl.ntiles / 8;
+        tbase = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+        tstride = G / 8;
+        tend = (int64_t)(blockIdx.x % 8 + 1) * per;
+    }
+    // the six field values of staged node `tid` of tile tt, into registers:
+    // the next tile's are read before this tile's flush, so their latency
+    // hides behind the flush's atomics
+    float nf[6];
+    auto fetch = [&](int64_t tt) {
+        const int ti = (int)(tt % tl.ntx);
+        const int tj = (int)((tt / tl.ntx) % tl.nty);
+        const int tk = (int)(tt / ((int64_t)tl.ntx * tl.nty));
+        const int li = tid % FX;
+        const int r = tid / FX;
         const int lj = r % FY;
         const int lk = r / FY;
-        const int gi = wrap1(i0 - 1 + li, a.nx);
-        const int gj = wrap1(j0 - 1 + lj, a.ny);
-        const int gk = wrap1(k0 - 1 + lk, a.nz);
+        const int gi = wrap1(ti * TX - 1 + li, a.nx);
+        const int gj = wrap1(tj * TY - 1 + lj, a.ny);
+        const int gk = wrap1(tk * TZ - 1 + lk, a.nz);
         const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
-        // two node-major arrays of 16-byte entries: (Ex, Ey, Ez, Bx) of the
-        // node, and (By, Bz) of the node and of its +x neighbour
-        sf4[e] = make_float4(a.f6[0][g], a.f6[1][g], a.f6[2][g], a.f6[3][g]);
-        const float2 byz = make_float2(a.f6[4][g], a.f6[5][g]);
-        float2* half = reinterpret_cast<float2*>(sfp);
-        half[2 * e] = byz;
-        if (e > 0) half[2 * e - 1] = byz;
-    }
-    for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
-    for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
-    if (tid < 64) hist[tid] = 0;
-    if (tid == 0) *nrec = 0u;
-    __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nf[q] = a.f6[q][g];
+    };
+    int64_t t = tbase;
+    if (t < tend && tid < fplane) fetch(t);
+    int i0 = 0, j0 = 0, k0 = 0;
 
     const LdsSink snk{sj, JX, JY, jplane};
-    const int lane = __lane_id();
-    const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint64_t ltmask = 0;
+    int lane = __lane_id();
     // tile of a (wrapped, so non-negative) cell: the tile extents are powers
     // of two in the default tile, and a grid of fewer than 2^31 cells has
     // fewer than 2^23 such tiles, so 32-bit arithmetic holds the index
@@ -762,6 +769,35 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         if (active) emit<3, HAS_CH, HAS_ID>(sp, a.pk, slots, td, slot, p);
     };
 
+#pragma unroll 1
+    for (; t < tend; t += tstride) {
+    // the thread and lane indices are opaque to the compiler from here on,
+    // so what the tile's phases derive from them is computed per tile
+    // instead of being hoisted out of the tile loop and held in registers
+    // through every phase
+    asm volatile("" : "+v"(tid), "+v"(lane));
+    ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    {
+        const int ti = (int)(t % tl.ntx);
+        const int tj = (int)((t / tl.ntx) % tl.nty);
+        const int tk = (int)(t / ((int64_t)tl.ntx * tl.nty));
+        i0 = ti * TX; j0 = tj * TY; k0 = tk * TZ;
+    }
+    if (tid < fplane) {
+        // two node-major arrays of 16-byte entries: (Ex, Ey, Ez, Bx) of the
+        // node, and (By, Bz) of the node and of its +x neighbour
+        sf4[tid] = make_float4(nf[0], nf[1], nf[2], nf[3]);
+        const float2 byz = make_float2(nf[4], nf[5]);
+        float2* half = reinterpret_cast<float2*>(sfp);
+        half[2 * tid] = byz;
+        if (tid > 0) half[2 * tid - 1] = byz;
+    }
+    for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
+    for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
+    if (tid < 64) hist[tid] = 0;
+    if (tid == 0) *nrec = 0u;
+    __syncthreads();
+
     if constexpr (!MULTI) {
 #pragma unroll 1
     for (int s = 0; s < 2; ++s) {
@@ -813,6 +849,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     if (s >= 2) break;
     }
     }
+    if (t + tstride < tend && tid < fplane) fetch(t + tstride);
     for (int e = tid; e < jplane; e += NT) {
         const int li = e % JX;
         const int r = e / JX;
@@ -827,6 +864,9 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             const float v = (float)sj[q * jplane + e];
             if (v != 0.0f) atomicAdd(a.j3[q] + g, v);
         }
+    }
+    // the next tile's stage overwrites the currents and the fields
+    __syncthreads();
     }
 }
 
@@ -1499,7 +1539,17 @@ int particles_push(tmpg_ctx* ctx, int mode)
                         k = vay ? k_push_cells<Ov, Cv, Iv, true, true, 8, 8, 4>
                                 : k_push_cells<Ov, Cv, Iv, true, false, 8, 8, 4>;
                 (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)CL.bytes);
-                hipLaunchKernelGGL(k, dim3((unsigned)tl.ntiles), dim3(CELLS_T), CL.bytes, ctx->stream, a, CL);
+                // persistent workgroups: as many as are resident at once
+                int per_cu = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, CELLS_T, CL.bytes) != hipSuccess ||
+                    per_cu < 1) {
+                    (void)hipGetLastError();
+                    per_cu = 1;
+                }
+                const int64_t grid = std::min<int64_t>(tl.ntiles, (int64_t)ctx->ncu * per_cu);
+                TiledArgs ac = a;
+                ac.remap = (grid % 8 == 0 && tl.ntiles % 8 == 0) ? 1 : 0;
+                hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(CELLS_T), CL.bytes, ctx->stream, ac, CL);
             } else {
                 auto k = k_push_tiled<Dv, Ov, Cv, Iv>;
                 (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -189,6 +189,7 @@ int tmpg_create(const tmpg_config* cfg, tmpg_ctx** out)
             else
                 (void)hipGetLastError();
         }
+        if (ncu > 0) ctx->ncu = ncu;
         int tx = cfg->tile[0] > 0 ? cfg->tile[0] : (cfg->dim == 3 ? 8 : 16);
         int ty = cfg->tile[1] > 0 ? cfg->tile[1] : (cfg->dim == 3 ? 8 : 16);
         if (cfg->dim == 2 && cfg->tile[1] <= 0 &&

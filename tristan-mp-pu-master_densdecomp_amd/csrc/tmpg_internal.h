@@ -126,6 +126,7 @@ struct tmpg_ctx {
     hipEvent_t* push_events = nullptr;  // set during tmpg_step: recorded around the tiled push kernel
     bool push_recorded = false;         // the push recorded push_events in this step
     int sort_every = 4;                 // tmpg_step sorts each tile's particles by cell every this many steps
+    int ncu = 256;                      // compute units of the context's device
     int steps_since_sort = 0;
     std::string err;
 };
