@@ -335,6 +335,35 @@ def push_counters(workload, kernel, npart, timeout=240):
             "waves_per_launch": mean["SQ_WAVES"], "cycles_per_launch": cycles}
 
 
+PHASES = ["stage", "push+records", "crossers+scans", "index scatter", "cell sums", "reset", "flush"]
+
+
+def phase_split(workload, steps, warmup):
+    """Per-step split of the 3D push kernel's tile time by phase, from the
+    phase-stamped build's cycle stamps (thread 0 of each workgroup, at the
+    phases' closing barriers)."""
+    import tmpg
+    tmpg.load_library(os.path.join(os.path.dirname(tmpg.LIB_PATH), "libtmpg_phases.so"))
+    w = WORKLOADS[workload]
+    ctx, _ = make_context(w)
+    ctx.step(warmup)
+    ctx.sync()
+    tmpg.phase_cycles(reset=True)
+    rows = []
+    for k in range(steps):
+        ctx.step(1)
+        ctx.sync()
+        cyc, cnt = tmpg.phase_cycles(reset=True)
+        tiles = max(int(cnt[0]), 1)
+        tot = float(sum(int(c) for c in cyc[:7]))
+        per = {PHASES[i]: int(cyc[i]) / tiles for i in range(7)}
+        rows.append(per)
+        print(f"step {warmup + k + 1}: " + ", ".join(f"{n} {per[n]:.0f} ({100 * int(cyc[i]) / tot:.1f}%)"
+                                                     for i, n in enumerate(PHASES)), flush=True)
+    ctx.close()
+    print(json.dumps({"workload": workload, "cycles_per_tile": rows}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,7 +380,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 times): the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--phase-split", type=int, default=0, metavar="STEPS",
+                    help="only run STEPS steps of the workload with the phase-stamped library "
+                         "(lib/libtmpg_phases.so) and print the push kernel's cycles per tile and phase")
     a = ap.parse_args()
+    if a.phase_split:
+        phase_split(a.workload, a.phase_split, a.warmup)
+        return
     if a.copy_calibration:
         import torch
         n = a.copy_calibration * (1 << 20) // 4

@@ -258,6 +258,16 @@ int tmpg_load_uniform(tmpg_ctx* ctx, int species, int ppc, float drift_gb,
  * [5] the tiled push+deposit kernel alone (part of [1]) */
 int tmpg_last_timings(tmpg_ctx* ctx, double ms[6]);
 
+/* per-phase shader-cycle sums of the 3D cell-sum push kernel, summed over
+ * its workgroups' tiles since the last reset (no reference counterpart: a
+ * profiling aid).  Phases: [0] field staging, [1] push, emit and deposit
+ * records, [2] y/z crosser deposits and scans, [3] record index scatter,
+ * [4] per-cell current sums, [5] record reset, [6] flush to the global
+ * currents; counts[] holds the number of stamps per phase.  Returns
+ * TMPG_ESTATE unless the library was built with phase stamps (the package
+ * build's --phase-stamps option, which writes lib/libtmpg_phases.so). */
+int tmpg_phase_cycles(unsigned long long cycles[8], unsigned long long counts[8], int reset);
+
 /* learned current denoiser (no reference counterpart: takes the place of
  * the apply_filter1_opt passes in low particles-per-cell runs) ------------
  * A periodic CNN (2D: 4 layers 3x3, width 32; 3D: 3 layers 3x3x3, width 16,

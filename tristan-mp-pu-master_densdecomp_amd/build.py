@@ -36,27 +36,32 @@ def sources():
     return sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def compile_one(src):
-    obj = os.path.join(OBJ, src.replace(".hip", ".o"))
-    cmd = [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+def compile_one(src, objdir=OBJ, defines=()):
+    obj = os.path.join(objdir, src.replace(".hip", ".o"))
+    cmd = [HIPCC] + HIPFLAGS + [f"-D{d}" for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build_lib(jobs=8):
-    os.makedirs(OBJ, exist_ok=True)
+def build_lib(jobs=8, phase_stamps=False):
+    """libtmpg.so; with phase_stamps, the profiling build libtmpg_phases.so
+    (per-phase cycle stamps in the 3D push, read by tmpg_phase_cycles)."""
+    objdir = os.path.join(OBJ, "phases") if phase_stamps else OBJ
+    lib = os.path.join(LIBDIR, "libtmpg_phases.so") if phase_stamps else LIB
+    defines = ("TMPG_PHASE_STAMPS",) if phase_stamps else ()
+    os.makedirs(objdir, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(compile_one, sources()))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda f: compile_one(f, objdir, defines), sources()))
+    tmp = lib + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def build_fortran():
@@ -86,9 +91,13 @@ def build_fortran():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--fortran", action="store_true")
+    ap.add_argument("--phase-stamps", action="store_true",
+                    help="also build lib/libtmpg_phases.so, the push with per-phase cycle stamps")
     ap.add_argument("-j", type=int, default=8)
     a = ap.parse_args()
     print(build_lib(a.j))
+    if a.phase_stamps:
+        print(build_lib(a.j, phase_stamps=True))
     if a.fortran:
         print(build_fortran())
 

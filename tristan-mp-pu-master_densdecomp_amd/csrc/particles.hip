@@ -27,6 +27,31 @@
 #include "tmpg_internal.h"
 
 namespace tmpg {
+
+// Per-phase cycle stamps of the cell-sum push (a profiling build only, built
+// with -DTMPG_PHASE_STAMPS): thread 0 of each workgroup adds the shader
+// cycles since its previous stamp to the phase's sum at every phase's
+// closing barrier.  Phases: 0 stage, 1 push/emit/records (phase A), 2 y/z
+// crossers and scans, 3 record index scatter, 4 per-cell sums (phase C),
+// 5 record reset, 6 flush to the global currents.
+#ifdef TMPG_PHASE_STAMPS
+__device__ unsigned long long g_phase_cycles[8];
+__device__ unsigned long long g_phase_count[8];
+#define PHASE_STAMP_START() unsigned long long phase_t0 = clock64()
+#define PHASE_STAMP(n)                                               \
+    do {                                                             \
+        if (threadIdx.x == 0) {                                      \
+            const unsigned long long phase_now = clock64();          \
+            atomicAdd(&g_phase_cycles[(n)], phase_now - phase_t0);   \
+            atomicAdd(&g_phase_count[(n)], 1ull);                    \
+            phase_t0 = phase_now;                                    \
+        }                                                            \
+    } while (0)
+#else
+#define PHASE_STAMP_START() do {} while (0)
+#define PHASE_STAMP(n) do {} while (0)
+#endif
+
 namespace {
 
 struct SpecArgs {
@@ -568,40 +593,64 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     int tid = threadIdx.x;
     constexpr int NT = CELLS_T;
     static_assert(fplane <= NT, "one staged node per thread");
+    PHASE_STAMP_START();
     // persistent workgroups: workgroup b takes the tiles tbase, tbase +
     // tstride, ... below tend.  With a.remap (the grid and the tile count
     // multiples of 8) the workgroups of one XCD (b mod 8) share one
     // contiguous eighth of the tiles, so neighbouring tiles' halos meet in
     // that XCD's L2.
+    // (tile indices in 32 bits: fewer than 2^31 tiles of 256 cells)
     const int G = gridDim.x;
-    int64_t tbase = blockIdx.x, tstride = G, tend = tl.ntiles;
+    const int ntiles = (int)tl.ntiles;
+    int tbase = blockIdx.x, tstride = G, tend = ntiles;
     if (a.remap) {
-        const int64_t per = tl.ntiles / 8;
-        tbase = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+        const int per = ntiles / 8;
+        tbase = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
         tstride = G / 8;
-        tend = (int64_t)(blockIdx.x % 8 + 1) * per;
+        tend = (int)(blockIdx.x % 8 + 1) * per;
     }
-    // the six field values of staged node `tid` of tile tt, into registers:
-    // the next tile's are read before this tile's flush, so their latency
-    // hides behind the flush's atomics
+    // tile tt's first cell
+    auto tile_origin = [&](int tt, int& x0, int& y0, int& z0) {
+        const unsigned u = (unsigned)tt, ntx = (unsigned)tl.ntx, nty = (unsigned)tl.nty;
+        const unsigned q = u / ntx;
+        x0 = (int)(u - q * ntx) * TX;
+        y0 = (int)(q % nty) * TY;
+        z0 = (int)(q / nty) * TZ;
+    };
+    // the six field values of staged node `tid` of tile tt: the next tile's
+    // are read before this tile's flush and written to the (by then unused)
+    // field arrays after it, so their latency hides behind the flush's
+    // atomics
     float nf[6];
-    auto fetch = [&](int64_t tt) {
-        const int ti = (int)(tt % tl.ntx);
-        const int tj = (int)((tt / tl.ntx) % tl.nty);
-        const int tk = (int)(tt / ((int64_t)tl.ntx * tl.nty));
-        const int li = tid % FX;
-        const int r = tid / FX;
+    auto fetch = [&](int tt) {
+        int x0, y0, z0;
+        tile_origin(tt, x0, y0, z0);
+        const int e = min(tid, fplane - 1);     // threads past the staged nodes read a valid one
+        const int li = e % FX;
+        const int r = e / FX;
         const int lj = r % FY;
         const int lk = r / FY;
-        const int gi = wrap1(ti * TX - 1 + li, a.nx);
-        const int gj = wrap1(tj * TY - 1 + lj, a.ny);
-        const int gk = wrap1(tk * TZ - 1 + lk, a.nz);
+        const int gi = wrap1(x0 - 1 + li, a.nx);
+        const int gj = wrap1(y0 - 1 + lj, a.ny);
+        const int gk = wrap1(z0 - 1 + lk, a.nz);
         const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
 #pragma unroll
         for (int q = 0; q < 6; ++q) nf[q] = a.f6[q][g];
     };
-    int64_t t = tbase;
-    if (t < tend && tid < fplane) fetch(t);
+    // two node-major arrays of 16-byte entries: (Ex, Ey, Ez, Bx) of the
+    // node, and (By, Bz) of the node and of its +x neighbour
+    auto put = [&]() {
+        sf4[tid] = make_float4(nf[0], nf[1], nf[2], nf[3]);
+        const float2 byz = make_float2(nf[4], nf[5]);
+        float2* half = reinterpret_cast<float2*>(sfp);
+        half[2 * tid] = byz;
+        if (tid > 0) half[2 * tid - 1] = byz;
+    };
+    int t = tbase;
+    if (t < tend && tid < fplane) {
+        fetch(t);
+        put();
+    }
     int i0 = 0, j0 = 0, k0 = 0;
 
     const LdsSink snk{sj, JX, JY, jplane};
@@ -655,6 +704,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             for (int c = lane6; c < nc; c += 64) atomicAdd(hist + min(cnt[c], 63), 1);
         }
         __syncthreads();
+        PHASE_STAMP(2);
         for (int r = tid; r < nr; r += NT) {
             const int c = rc[r];
             const int pos = atomicAdd(fill + c, 1);
@@ -675,6 +725,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             for (int c = lane6; c < nc; c += 64) perm[atomicAdd(hist + min(cnt[c], 63), 1)] = (uint16_t)c;
         }
         __syncthreads();
+        PHASE_STAMP(3);
 
         // ---- phase C: per-cell register sums, four adjacent lanes per cell
         // (one per z slot)
@@ -684,10 +735,12 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             cell_sums<ORD>(wk & 3, c, start[c], cnt[c], TX, TY, idx, rec, rq, snk);
         }
         __syncthreads();
+        PHASE_STAMP(4);
         for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
         if (tid < 64) hist[tid] = 0;
         if (tid == 0) *nrec = 0u;
         __syncthreads();
+        PHASE_STAMP(5);
     };
 
     // ---- phase A: push, emit, record or deposit at once.  A tile with more
@@ -722,7 +775,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         // wave's stayers one block in this tile.  Both global atomics are
         // issued here and their results used only after the deposit record
         // is written, so the record work covers their latency.
-        const bool stay = active && (int64_t)td == t;
+        const bool stay = active && td == t;
         const uint64_t m = __ballot(stay);
         int32_t lslot = 0, b0 = 0;
         if (active && !stay) lslot = atomicAdd(sp.cnt_b + td, 1);
@@ -777,33 +830,20 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     // through every phase
     asm volatile("" : "+v"(tid), "+v"(lane));
     ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    {
-        const int ti = (int)(t % tl.ntx);
-        const int tj = (int)((t / tl.ntx) % tl.nty);
-        const int tk = (int)(t / ((int64_t)tl.ntx * tl.nty));
-        i0 = ti * TX; j0 = tj * TY; k0 = tk * TZ;
-    }
-    if (tid < fplane) {
-        // two node-major arrays of 16-byte entries: (Ex, Ey, Ez, Bx) of the
-        // node, and (By, Bz) of the node and of its +x neighbour
-        sf4[tid] = make_float4(nf[0], nf[1], nf[2], nf[3]);
-        const float2 byz = make_float2(nf[4], nf[5]);
-        float2* half = reinterpret_cast<float2*>(sfp);
-        half[2 * tid] = byz;
-        if (tid > 0) half[2 * tid - 1] = byz;
-    }
+    tile_origin(t, i0, j0, k0);
     for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
     for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
     if (tid < 64) hist[tid] = 0;
     if (tid == 0) *nrec = 0u;
     __syncthreads();
+    PHASE_STAMP(0);
 
     if constexpr (!MULTI) {
 #pragma unroll 1
     for (int s = 0; s < 2; ++s) {
         const SpecArgs& sp = a.sp[s];
         const int n = min(sp.cnt_a[t], sp.tcap);
-        const int64_t base = t * sp.tcap;
+        const int64_t base = (int64_t)t * sp.tcap;
         const int64_t slots = (int64_t)tl.ntiles * sp.tcap;
         // the next round's particles are loaded while this round is pushed
         Part pn;
@@ -812,6 +852,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         for (int p0 = 0; p0 < n; p0 += NT) round(sp, n, base, slots, p0, pn, an);
     }
     __syncthreads();
+    PHASE_STAMP(1);
     drain();
     } else {
     // dense tiles: phase A stops when a round might not fit, the records
@@ -826,7 +867,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     for (; s < 2; ++s, p0 = 0) {
         const SpecArgs& sp = a.sp[s];
         const int n = min(sp.cnt_a[t], sp.tcap);
-        const int64_t base = t * sp.tcap;
+        const int64_t base = (int64_t)t * sp.tcap;
         const int64_t slots = (int64_t)tl.ntiles * sp.tcap;
         Part pn;
         bool an = p0 + tid < n;
@@ -845,12 +886,24 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         if (full) break;
     }
     __syncthreads();
+    PHASE_STAMP(1);
     drain();
     if (s >= 2) break;
     }
     }
-    if (t + tstride < tend && tid < fplane) fetch(t + tstride);
-    for (int e = tid; e < jplane; e += NT) {
+    // (the flush's addresses are computed from here on, not carried from
+    // the earlier phases)
+    asm volatile("" : "+v"(tid));
+    // (read unconditionally, the last tile's own fields again when there is
+    // no next tile, so that the values are not held through the flush's
+    // branches in scratch)
+    const bool more = t + tstride < tend;
+    fetch(more ? t + tstride : t);
+    // (a fixed trip count, unrolled: the fetched values stay in registers)
+#pragma unroll
+    for (int it = 0; it < (jplane + NT - 1) / NT; ++it) {
+        const int e = tid + it * NT;
+        if (e >= jplane) break;
         const int li = e % JX;
         const int r = e / JX;
         const int lj = r % JY;
@@ -865,8 +918,10 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             if (v != 0.0f) atomicAdd(a.j3[q] + g, v);
         }
     }
-    // the next tile's stage overwrites the currents and the fields
+    if (more && tid < fplane) put();
+    // the next tile's stage clears the currents
     __syncthreads();
+    PHASE_STAMP(6);
     }
 }
 
@@ -1612,3 +1667,25 @@ int particles_push(tmpg_ctx* ctx, int mode)
 }
 
 }  // namespace tmpg
+
+// the per-phase cycle sums of the cell-sum push (profiling build only)
+extern "C" int tmpg_phase_cycles(unsigned long long cycles[8], unsigned long long counts[8], int reset)
+{
+    if (!cycles || !counts) return TMPG_EINVAL;
+#ifdef TMPG_PHASE_STAMPS
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpyFromSymbol(cycles, HIP_SYMBOL(tmpg::g_phase_cycles), sizeof(unsigned long long) * 8) != hipSuccess ||
+        hipMemcpyFromSymbol(counts, HIP_SYMBOL(tmpg::g_phase_count), sizeof(unsigned long long) * 8) != hipSuccess)
+        return TMPG_EHIP;
+    if (reset) {
+        const unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(tmpg::g_phase_cycles), z, sizeof(z)) != hipSuccess ||
+            hipMemcpyToSymbol(HIP_SYMBOL(tmpg::g_phase_count), z, sizeof(z)) != hipSuccess)
+            return TMPG_EHIP;
+    }
+    return TMPG_OK;
+#else
+    (void)reset;
+    return TMPG_ESTATE;     // built without phase stamps
+#endif
+}

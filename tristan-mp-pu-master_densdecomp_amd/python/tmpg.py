@@ -31,7 +31,7 @@ EXPORTS = [
     "tmpg_advance_b_half", "tmpg_advance_e", "tmpg_reset_currents", "tmpg_move", "tmpg_deposit",
     "tmpg_push_deposit", "tmpg_fold_currents", "tmpg_filter", "tmpg_add_current", "tmpg_step",
     "tmpg_diag", "tmpg_gauss_residual", "tmpg_gauss_residual_noalloc", "tmpg_charge_density", "tmpg_track_continuity", "tmpg_load_uniform",
-    "tmpg_last_timings", "tmpg_input_open", "tmpg_input_close", "tmpg_input_get_int",
+    "tmpg_last_timings", "tmpg_phase_cycles", "tmpg_input_open", "tmpg_input_close", "tmpg_input_get_int",
     "tmpg_input_get_double",
     "tmpg_model_create", "tmpg_model_destroy", "tmpg_model_last_error", "tmpg_model_num_params",
     "tmpg_model_get_params", "tmpg_model_set_params", "tmpg_model_set_scale", "tmpg_model_get_scale",
@@ -352,6 +352,20 @@ class Context:
         out = np.zeros(6, dtype=np.float64)
         self._chk(self.lib.tmpg_last_timings(self.h, _p(out)), "last_timings")
         return out
+
+
+def phase_cycles(reset=True):
+    """Per-phase shader-cycle sums and stamp counts of the 3D cell-sum push
+    (tmpg_phase_cycles); raises unless the loaded library is the
+    phase-stamped build (lib/libtmpg_phases.so)."""
+    lib = load_library()
+    cyc = np.zeros(8, dtype=np.uint64)
+    cnt = np.zeros(8, dtype=np.uint64)
+    rc = lib.tmpg_phase_cycles(_p(cyc), _p(cnt), int(reset))
+    if rc != 0:
+        raise TmpgError(f"tmpg_phase_cycles failed ({rc}): the library has no phase stamps" if rc == -5
+                        else f"tmpg_phase_cycles failed ({rc})")
+    return cyc, cnt
 
 
 class Model:
