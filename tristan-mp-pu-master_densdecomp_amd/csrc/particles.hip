@@ -666,7 +666,17 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     // ---- the records so far: crossers deposited directly, the rest
     // counting-sorted by cell and summed per cell (phases B and C); the
     // record space is then empty again
-    auto drain = [&]() {
+    // (last: the tile's final drain: the fields are no longer read, so the
+    // next tile's are read at its start and stored after the crossers)
+    bool more = false;
+    auto drain = [&](bool last) {
+        if (last) {
+            // read unconditionally (the last tile's own fields again when
+            // there is no next tile), so that the values are not held in
+            // scratch through the crossers' branches
+            more = t + tstride < tend;
+            fetch(more ? t + tstride : t);
+        }
         // ---- direct deposit of the y/z crossers, one z node plane of one
         // crosser per thread (so every wave shares the work)
         const int nr = (int)(*nrec & 0xffffu);
@@ -705,6 +715,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         }
         __syncthreads();
         PHASE_STAMP(2);
+        if (last && more && tid < fplane) put();
         for (int r = tid; r < nr; r += NT) {
             const int c = rc[r];
             const int pos = atomicAdd(fill + c, 1);
@@ -853,7 +864,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     }
     __syncthreads();
     PHASE_STAMP(1);
-    drain();
+    drain(true);
     } else {
     // dense tiles: phase A stops when a round might not fit, the records
     // are drained, and phase A resumes at the block-uniform point
@@ -887,18 +898,13 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     }
     __syncthreads();
     PHASE_STAMP(1);
-    drain();
+    drain(s >= 2);
     if (s >= 2) break;
     }
     }
     // (the flush's addresses are computed from here on, not carried from
     // the earlier phases)
     asm volatile("" : "+v"(tid));
-    // (read unconditionally, the last tile's own fields again when there is
-    // no next tile, so that the values are not held through the flush's
-    // branches in scratch)
-    const bool more = t + tstride < tend;
-    fetch(more ? t + tstride : t);
     // (a fixed trip count, unrolled: the fetched values stay in registers)
 #pragma unroll
     for (int it = 0; it < (jplane + NT - 1) / NT; ++it) {
@@ -918,7 +924,6 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             if (v != 0.0f) atomicAdd(a.j3[q] + g, v);
         }
     }
-    if (more && tid < fplane) put();
     // the next tile's stage clears the currents
     __syncthreads();
     PHASE_STAMP(6);
