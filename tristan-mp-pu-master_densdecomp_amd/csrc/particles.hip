@@ -703,11 +703,14 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
                 int x = v;
     #pragma unroll
                 for (int o = 1; o < 64; o <<= 1) {
-                    const int y = __shfl_up(x, o, 64);
+                    // lane lane6 - o's x (the lane index comes from the
+                    // per-tile thread index, so no address is held across
+                    // the tile loop)
+                    const int y = __builtin_amdgcn_ds_bpermute((lane6 - o) << 2, x);
                     if (lane6 >= o) x += y;
                 }
                 if (c < nc) start[c] = carry + x - v;
-                carry += __shfl(x, 63, 64);
+                carry += __builtin_amdgcn_readlane(x, 63);
             }
         } else if (wv == nwv - 2) {
             // histogram of the record counts (bins 0..63)
@@ -728,7 +731,7 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             int x = h;
     #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_down(x, o, 64);
+                const int y = __builtin_amdgcn_ds_bpermute((lane6 + o) << 2, x);
                 if (lane6 + o < 64) x += y;
             }
             hist[lane6] = x - h;
