@@ -651,6 +651,14 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         fetch(t);
         put();
     }
+    // the currents and record counters start cleared; afterwards the flush
+    // clears each current entry it reads and the tile's final drain clears
+    // the counters, so a tile needs no clearing pass of its own
+    for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
+    for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
+    if (tid < 64) hist[tid] = 0;
+    if (tid == 0) *nrec = 0u;
+    __syncthreads();
     int i0 = 0, j0 = 0, k0 = 0;
 
     const LdsSink snk{sj, JX, JY, jplane};
@@ -753,7 +761,9 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
         if (tid < 64) hist[tid] = 0;
         if (tid == 0) *nrec = 0u;
-        __syncthreads();
+        // after the tile's final drain the barrier closing the tile orders
+        // these stores before the next tile's phase A
+        if (!last) __syncthreads();
         PHASE_STAMP(5);
     };
 
@@ -845,11 +855,6 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     asm volatile("" : "+v"(tid), "+v"(lane));
     ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     tile_origin(t, i0, j0, k0);
-    for (int e = tid; e < 3 * jplane; e += NT) sj[e] = 0.0;
-    for (int e = tid; e < nc; e += NT) { cnt[e] = 0; fill[e] = 0; }
-    if (tid < 64) hist[tid] = 0;
-    if (tid == 0) *nrec = 0u;
-    __syncthreads();
     PHASE_STAMP(0);
 
     if constexpr (!MULTI) {
@@ -924,10 +929,12 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const float v = (float)sj[q * jplane + e];
+            sj[q * jplane + e] = 0.0;       // cleared for the next tile
             if (v != 0.0f) atomicAdd(a.j3[q] + g, v);
         }
     }
-    // the next tile's stage clears the currents
+    // closes the tile: the fields, the cleared currents and counters are
+    // in place for the next tile's phase A
     __syncthreads();
     PHASE_STAMP(6);
     }
