@@ -466,6 +466,10 @@ __device__ __forceinline__ void deposit3_plane(const Sink& snk, int i, int j, in
             cz = q * run_z;
         }
     }
+    // a plane at the end of the window that the particle's z shape does not
+    // reach adds nothing (a move that does not cross in z leaves one such
+    // plane of the five)
+    if (az == 0.0f && dsz == 0.0f && (cz == 0.0f || zc == W - 1)) return;
     const float twelfth = 1.0f / 12.0f;
     const float dszt = twelfth * dsz;
     // the plane's weights, each a fused multiply-add: wyz(b) = Ay az + dSy
@@ -487,8 +491,11 @@ __device__ __forceinline__ void deposit3_plane(const Sink& snk, int i, int j, in
         const float wyz = fmaf(Ay[b], az, dSy[b] * dszt);
 #pragma unroll
         for (int t = 0; t < W; ++t) {
-            if (t < W - 1) snk.add(0, r, cc[t], Cx[t] * wyz);
-            if (b < W - 1) snk.add(1, r, cc[t], Cy[b] * wxz[t]);
+            // (the window's end nodes on an axis the move does not cross
+            // carry exact zeros, which need no atomic)
+            const float vx = Cx[t] * wyz, vy = Cy[b] * wxz[t];
+            if (t < W - 1 && vx != 0.0f) snk.add(0, r, cc[t], vx);
+            if (b < W - 1 && vy != 0.0f) snk.add(1, r, cc[t], vy);
         }
     }
     // the last plane's z prefix is the telescoped zero
@@ -497,7 +504,10 @@ __device__ __forceinline__ void deposit3_plane(const Sink& snk, int i, int j, in
         for (int b = 0; b < W; ++b) {
             const auto r = snk.row(j + by + b, k + bz + zc);
 #pragma unroll
-            for (int t = 0; t < W; ++t) snk.add(2, r, cc[t], fmaf(gd[t], dSy[b], ga[t] * Ay[b]));
+            for (int t = 0; t < W; ++t) {
+                const float vz = fmaf(gd[t], dSy[b], ga[t] * Ay[b]);
+                if (vz != 0.0f) snk.add(2, r, cc[t], vz);
+            }
         }
     }
 }
