@@ -539,25 +539,30 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
             }
         }
     }
-    // add the cell's sums to the tile currents: node (li-2+u, lj-1+b,
+    // add the cell's sums to the tile currents (the window's outer nodes
+    // collect only from records that cross there, so many sums are exact
+    // zeros, which need no atomic): node (li-2+u, lj-1+b,
     // lk-1+Q) sits at LDS (li+u, lj+1+b, lk+1+Q)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const int row = snk.row(lj + 1 + b, lk + 1 + Q);
 #pragma unroll
-        for (int u = 0; u < 5; ++u) snk.add(0, row, li + u, ax[u][b]);
+        for (int u = 0; u < 5; ++u)
+            if (ax[u][b] != 0.0f) snk.add(0, row, li + u, ax[u][b]);
     }
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
         const int row = snk.row(lj + 1 + b, lk + 1 + Q);
 #pragma unroll
-        for (int u = 0; u < 6; ++u) snk.add(1, row, li + u, ay[u][b]);
+        for (int u = 0; u < 6; ++u)
+            if (ay[u][b] != 0.0f) snk.add(1, row, li + u, ay[u][b]);
     }
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         const int row = snk.row(lj + 1 + Q, lk + 1 + s);
 #pragma unroll
-        for (int u = 0; u < 6; ++u) snk.add(2, row, li + u, az[s][u]);
+        for (int u = 0; u < 6; ++u)
+            if (az[s][u] != 0.0f) snk.add(2, row, li + u, az[s][u]);
     }
 }
 
