@@ -918,26 +918,39 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
     // (the flush's addresses are computed from here on, not carried from
     // the earlier phases)
     asm volatile("" : "+v"(tid));
-    // (a fixed trip count, unrolled: the fetched values stay in registers)
+    // a thread's (at most two) nodes: all their values are read from LDS
+    // and cleared for the next tile first, then the atomics are issued, so
+    // the LDS reads overlap instead of each waiting before its atomic
+    constexpr int FI = (jplane + NT - 1) / NT;
+    float fv[FI][3];
+    int64_t fg[FI];
 #pragma unroll
-    for (int it = 0; it < (jplane + NT - 1) / NT; ++it) {
+    for (int it = 0; it < FI; ++it) {
         const int e = tid + it * NT;
-        if (e >= jplane) break;
-        const int li = e % JX;
-        const int r = e / JX;
-        const int lj = r % JY;
-        const int lk = r / JY;
-        const int gi = wrap1(i0 - 2 + li, a.nx);
-        const int gj = wrap1(j0 - 2 + lj, a.ny);
-        const int gk = wrap1(k0 - 2 + lk, a.nz);
-        const int64_t g = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
+        fg[it] = -1;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const float v = (float)sj[q * jplane + e];
-            sj[q * jplane + e] = 0.0;       // cleared for the next tile
-            if (v != 0.0f) atomicAdd(a.j3[q] + g, v);
+        for (int q = 0; q < 3; ++q) fv[it][q] = 0.0f;
+        if (e < jplane) {
+            const int li = e % JX;
+            const int r = e / JX;
+            const int lj = r % JY;
+            const int lk = r / JY;
+            const int gi = wrap1(i0 - 2 + li, a.nx);
+            const int gj = wrap1(j0 - 2 + lj, a.ny);
+            const int gk = wrap1(k0 - 2 + lk, a.nz);
+            fg[it] = gi + (int64_t)a.nx * (gj + (int64_t)a.ny * gk);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                fv[it][q] = (float)sj[q * jplane + e];
+                sj[q * jplane + e] = 0.0;
+            }
         }
     }
+#pragma unroll
+    for (int it = 0; it < FI; ++it)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (fv[it][q] != 0.0f) atomicAdd(a.j3[q] + fg[it], fv[it][q]);
     // closes the tile: the fields, the cleared currents and counters are
     // in place for the next tile's phase A
     __syncthreads();
