@@ -633,6 +633,180 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(Geo g, const uint16_t* __res
     }
 }
 
+// dW[co][k] += sum_p dz[p][co] in[nb(p, tap(k))][ci(k)], db[co] += sum_p
+// dz[p][co], on v_mfma_f32_16x16x32_bf16: M = 16 output channels, N = 16
+// weight columns (tap, input channel), K = 32 pixels of a tile.  Persistent
+// workgroups of 8 waves walk the forward convolution's 512-pixel tiles; per
+// tile they stage the haloed input (pixel-major, as the forward does) and
+// dz transposed (channel-major, so a lane's 8 pixels are one 16-byte read)
+// in LDS.  Wave w owns the (co tile, column tile) pairs w, w + 8, ... (at
+// most WG_MT of them) and sums them in registers over all its tiles.  Each
+// workgroup then writes its sums (and its bias sums) to its own slot of
+// part, [grid][cout_pad * kpad + cout_pad]; k_wgrad_reduce adds the slots
+// to dW and db.  (Atomics from every workgroup onto the same few KB of dW
+// serialise on their cache lines and took most of the time.)
+constexpr int WG_THREADS = 512, WG_MT = 5;
+template <int DIM, int CIN>
+__host__ __device__ constexpr size_t wgrad_lds_bytes(int cout_pad)
+{
+    return sizeof(uint16_t) * ((size_t)conv_in_halves<DIM, 1, CIN>() +
+                               (size_t)cout_pad * (ConvTile<DIM, 1>::NPIX + 8));
+}
+
+template <int DIM, int CIN>
+__global__ void __launch_bounds__(WG_THREADS) k_wgrad(Geo g, const uint16_t* __restrict__ dz, int dz_stride,
+                                                      int cout, int cout_pad, const uint16_t* __restrict__ in,
+                                                      int taps, int kpad, float* __restrict__ part)
+{
+    using T = ConvTile<DIM, 1>;
+    constexpr int NPIX = T::NPIX, PS = conv_pixel_stride(CIN), DS = NPIX + 8;
+    constexpr int HALO = T::HX * T::HY * T::HZ, UPP = CIN / 4;     // 8-byte units per staged pixel
+    static_assert(NPIX % 32 == 0 && T::TX % 8 == 0, "32-pixel steps of whole 8-pixel row pieces");
+    extern __shared__ uint16_t sm[];
+    uint16_t* sin = sm;
+    uint16_t* sdz = sm + conv_in_halves<DIM, 1, CIN>();
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, kq = lane >> 4;
+    const int K = taps * CIN, nkt = kpad / 16, ntile = (cout_pad / 16) * nkt;
+    const int ntx = (g.nx + T::TX - 1) / T::TX, nty = (g.ny + T::TY - 1) / T::TY;
+    const int ntz = (g.nz + T::TZ - 1) / T::TZ;
+    const int64_t ntiles = (int64_t)ntx * nty * ntz * g.batch;
+    // this lane's A row (output channel) and B column (tap, input channel)
+    // of each owned pair: the column's offset in the staged input, in
+    // halfwords, relative to the pixel's own position
+    int aco[WG_MT], boff[WG_MT];
+    bool bval[WG_MT];
+    f32x4 acc[WG_MT];
+#pragma unroll
+    for (int m = 0; m < WG_MT; ++m) {
+        const int j = wave + 8 * m;
+        const int ct = j / nkt, kt = j - ct * nkt;
+        const int k = kt * 16 + i;
+        aco[m] = ct * 16 + i;
+        bval[m] = j < ntile && k < K;
+        const int tap = bval[m] ? k / CIN : 0, ci = k - tap * CIN;
+        const int dx = tap % 3, dy = (tap / 3) % 3, dzo = DIM == 3 ? tap / 9 : 0;   // offsets + 1
+        boff[m] = (dx + T::HX * (dy + T::HY * dzo)) * PS + ci;
+        acc[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    float bsum[2] = {0.0f, 0.0f};
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int64_t r = t;
+        const int x0 = (int)(r % ntx) * T::TX;
+        r /= ntx;
+        const int y0 = (int)(r % nty) * T::TY;
+        r /= nty;
+        const int z0 = (int)(r % ntz) * T::TZ;
+        const int64_t ibase = (r / ntz) * g.npix;
+        __syncthreads();    // the previous tile's reads are done
+        for (int u = tid; u < HALO * UPP; u += WG_THREADS) {
+            const int hp = u / UPP, uc = u - hp * UPP;
+            const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
+            const int x = wrap_once(x0 + hx - 1, g.nx), y = wrap_once(y0 + hy - 1, g.ny);
+            const int z = DIM == 3 ? wrap_once(z0 + hz - 1, g.nz) : 0;
+            const int64_t q = ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+            *reinterpret_cast<u32x2*>(sin + hp * PS + 4 * uc) =
+                *reinterpret_cast<const u32x2*>(in + q * CIN + 4 * uc);
+        }
+        for (int u = tid; u < NPIX * cout_pad; u += WG_THREADS) {
+            const int p = u / cout_pad, co = u - p * cout_pad;
+            const int x = x0 + p % T::TX, y = y0 + (p / T::TX) % T::TY, z = z0 + p / (T::TX * T::TY);
+            uint16_t v = 0;
+            if (co < cout && x < g.nx && y < g.ny && z < g.nz)
+                v = dz[(ibase + x + (int64_t)g.nx * (y + (int64_t)g.ny * z)) * dz_stride + co];
+            sdz[co * DS + p] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+            const int u = tid + WG_THREADS * jb;
+            if (u < cout_pad * 32) {
+                const int co = u >> 5, p0 = (u & 31) * (NPIX / 32);
+                float s = 0.0f;
+                for (int p = p0; p < p0 + NPIX / 32; ++p) s += bf2f(sdz[co * DS + p]);
+                bsum[jb] += s;
+            }
+        }
+        for (int st = 0; st < NPIX / 32; ++st) {
+            // this lane's 8 pixels: one piece of a tile row
+            const int p0 = st * 32 + 8 * kq;
+            const int px = p0 % T::TX, py = (p0 / T::TX) % T::TY, pz = p0 / (T::TX * T::TY);
+            const int hbase = (px + T::HX * (py + T::HY * pz)) * PS;
+#pragma unroll
+            for (int m = 0; m < WG_MT; ++m) {
+                if (wave + 8 * m >= ntile) break;
+                const bf16x8 a = bf16x8_of(*reinterpret_cast<const uint4*>(sdz + aco[m] * DS + p0));
+                uint4 bv = make_uint4(0u, 0u, 0u, 0u);
+                if (bval[m]) {
+                    const uint16_t* b = sin + hbase + boff[m];
+                    bv.x = (unsigned)b[0] | ((unsigned)b[PS] << 16);
+                    bv.y = (unsigned)b[2 * PS] | ((unsigned)b[3 * PS] << 16);
+                    bv.z = (unsigned)b[4 * PS] | ((unsigned)b[5 * PS] << 16);
+                    bv.w = (unsigned)b[6 * PS] | ((unsigned)b[7 * PS] << 16);
+                }
+                acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf16x8_of(bv), acc[m], 0, 0, 0);
+            }
+        }
+    }
+    // D[m][n]: n = lane % 16 (column), m = 4 (lane / 16) + r (output
+    // channel); the owned pairs cover the whole [cout_pad][kpad] slot
+    float* pw = part + (int64_t)blockIdx.x * ((int64_t)cout_pad * kpad + cout_pad);
+#pragma unroll
+    for (int m = 0; m < WG_MT; ++m) {
+        const int j = wave + 8 * m;
+        if (j >= ntile) break;
+        const int ct = j / nkt, kt = j - ct * nkt;
+        const int n = kt * 16 + i;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) pw[(ct * 16 + 4 * kq + rr) * kpad + n] = acc[m][rr];
+    }
+    // the 32 partial bias sums of each channel, added in LDS
+    __syncthreads();
+    float* sb = reinterpret_cast<float*>(sm);
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+        const int u = tid + WG_THREADS * jb;
+        if (u < cout_pad * 32) sb[u] = bsum[jb];
+    }
+    __syncthreads();
+    if (tid < cout_pad) {
+        float sum = 0.0f;
+        for (int q = 0; q < 32; ++q) sum += sb[tid * 32 + q];
+        pw[cout_pad * kpad + tid] = sum;
+    }
+}
+
+// dW[co][k] += sum over the nblk slots of part (k < K, co < cout), and
+// db[co] += the slots' bias sums, without atomics: a block of 256 threads
+// takes 16 values, 16 threads per value each summing every 16th slot, and
+// adds the 16 partial sums in LDS
+__global__ void __launch_bounds__(256) k_wgrad_reduce(const float* __restrict__ part, int nblk, int cout,
+                                                      int cout_pad, int kpad, int K, float* __restrict__ dW,
+                                                      float* __restrict__ db)
+{
+    __shared__ float red[16][17];
+    const int E = cout_pad * kpad, stride = E + cout_pad;
+    const int le = threadIdx.x & 15, sl = threadIdx.x >> 4;
+    const int e = blockIdx.x * 16 + le;
+    float sum = 0.0f;
+    if (e < stride)
+        for (int b = sl; b < nblk; b += 16) sum += part[(int64_t)b * stride + e];
+    red[sl][le] = sum;
+    __syncthreads();
+    if (threadIdx.x >= 16) return;
+    const int e2 = blockIdx.x * 16 + threadIdx.x;
+    float tot = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tot += red[q][threadIdx.x];
+    if (e2 < E) {
+        const int co = e2 / kpad, n = e2 - co * kpad;
+        if (co < cout && n < K) dW[(int64_t)co * kpad + n] += tot;
+    } else if (e2 < stride && db) {
+        const int co = e2 - E;
+        if (co < cout) db[co] += tot;
+    }
+}
+
 // x (BF16, 4 channels) = J / s from the three FP32 current arrays of a batch
 __global__ void k_pack_input(const float* __restrict__ j, int64_t n, int batch, float inv_s,
                              uint16_t* __restrict__ x)
@@ -895,6 +1069,8 @@ struct tmpg_model {
     float *jlo = nullptr, *jt = nullptr;
     double* dloss = nullptr;
     double* dots = nullptr;              // per image <d,t>, <t,t> (amplitude term)
+    float* wpart = nullptr;              // per-workgroup weight-gradient partial sums
+    int64_t wpart_cap = 0;               // (floats)
     float lam = 0.0f;                    // weight of the amplitude term
     // ordering between a context's stream and the model's stream in
     // denoise_apply (created once with the model)
@@ -1060,6 +1236,51 @@ int forward(tmpg_model* m, const Geo& g, bool keep_z = true)
 }
 
 // backward from m->gcurl (dL/dcurl): accumulates gw, gb
+// the tiled weight gradient k_wgrad for (dim, cin) in {2, 3} x {4, 16, 32}
+// with at most 8 * WG_MT output pairs; false if the shape is not one of them
+template <int D, int CIN>
+int launch_wgrad_t(tmpg_model* m, const Geo& g, int dz_c, int cout, int cout_pad, const uint16_t* in, int taps,
+                    int kpad, float* gw, float* gb)
+{
+    using T = ConvTile<D, 1>;
+    const size_t lds = wgrad_lds_bytes<D, CIN>(cout_pad);
+    (void)hipFuncSetAttribute((const void*)k_wgrad<D, CIN>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_wgrad<D, CIN>, WG_THREADS, lds) != hipSuccess ||
+        per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    const int64_t tiles = (int64_t)cdiv(g.nx, T::TX) * cdiv(g.ny, T::TY) * cdiv(g.nz, T::TZ) * g.batch;
+    const int64_t grid = std::min<int64_t>(tiles, (int64_t)m->ncu * per_cu);
+    const int64_t stride = (int64_t)cout_pad * kpad + cout_pad;
+    if (grid * stride > m->wpart_cap) {
+        if (m->wpart) (void)hipFree(m->wpart);
+        m->wpart = nullptr;
+        m->wpart_cap = 0;
+        if (dn_alloc(m, &m->wpart, grid * stride)) return TMPG_ENOMEM;
+        m->wpart_cap = grid * stride;
+    }
+    hipLaunchKernelGGL((k_wgrad<D, CIN>), dim3((unsigned)grid), dim3(WG_THREADS), lds, m->stream, g, m->dz, dz_c,
+                       cout, cout_pad, in, taps, kpad, m->wpart);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)cdiv(stride, 16)), dim3(256), 0, m->stream, m->wpart,
+                       (int)grid, cout, cout_pad, kpad, taps * CIN, gw, gb);
+    return TMPG_OK;
+}
+
+// returns 1 when the shape has no tiled kernel (the caller then runs the
+// per-chunk one), else TMPG_OK or an error code
+int launch_wgrad(tmpg_model* m, const Geo& g, int dz_c, int cout, int cout_pad, const uint16_t* in, int cin,
+                 int taps, int kpad, float* gw, float* gb)
+{
+    if (cout_pad % 16 || cout_pad > 32 || (cout_pad / 16) * (kpad / 16) > 8 * WG_MT || kpad % 16) return 1;
+    if (g.dim == 3 && cin == 4) return launch_wgrad_t<3, 4>(m, g, dz_c, cout, cout_pad, in, taps, kpad, gw, gb);
+    if (g.dim == 3 && cin == 16) return launch_wgrad_t<3, 16>(m, g, dz_c, cout, cout_pad, in, taps, kpad, gw, gb);
+    if (g.dim == 2 && cin == 4) return launch_wgrad_t<2, 4>(m, g, dz_c, cout, cout_pad, in, taps, kpad, gw, gb);
+    if (g.dim == 2 && cin == 32) return launch_wgrad_t<2, 32>(m, g, dz_c, cout, cout_pad, in, taps, kpad, gw, gb);
+    return 1;
+}
+
 int backward(tmpg_model* m, const Geo& g)
 {
     const int64_t P = g.npix * g.batch;
@@ -1072,9 +1293,14 @@ int backward(tmpg_model* m, const Geo& g)
         DLayer& L = m->L[l];
         const uint16_t* in = (l == 0) ? m->x0 : m->h[l - 1];
         const int cin = (l == 0) ? 4 : m->L[l - 1].cout;
-        // weight gradient (dz stored with stride dz_c; cout real channels)
-        hipLaunchKernelGGL(k_conv_wgrad, dim3((unsigned)cdiv(P, chunk)), dim3(256), 0, m->stream, g, m->dz, dz_c,
-                           L.cout, L.cout_pad, in, cin, L.taps, L.kpad, chunk, L.gw, L.gb);
+        // weight gradient (dz stored with stride dz_c; cout real channels):
+        // the tiled MFMA kernel for the model's layer shapes, the per-chunk
+        // one otherwise
+        const int wrc = launch_wgrad(m, g, dz_c, L.cout, L.cout_pad, in, cin, L.taps, L.kpad, L.gw, L.gb);
+        if (wrc < 0) return wrc;
+        if (wrc == 1)
+            hipLaunchKernelGGL(k_conv_wgrad, dim3((unsigned)cdiv(P, chunk)), dim3(256), 0, m->stream, g, m->dz,
+                               dz_c, L.cout, L.cout_pad, in, cin, L.taps, L.kpad, chunk, L.gw, L.gb);
         if (l == 0) break;
         // input gradient: conv of dz with the transposed, flipped weights
         const int rc = launch_conv(m, g, m->dz, dz_c, L.taps, L.wt, L.kpadT, L.cinT_pad, nullptr, L.cin, m->dh,
@@ -1253,6 +1479,7 @@ int tmpg_model_destroy(tmpg_model* m)
     for (auto p : m->set_lo) f(p);
     for (auto p : m->set_t) f(p);
     f(m->dloss);
+    f(m->wpart);
     f(m->dots);
     if (m->ev_in) (void)hipEventDestroy(m->ev_in);
     if (m->ev_out) (void)hipEventDestroy(m->ev_out);
