@@ -705,7 +705,8 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
         }
 
         // ---- phase B: counting sort of the cell-summed records by cell (the
-        // crossers above run on the first waves; the scans on the last two)
+        // crossers above run on the first waves; the scans and the cell
+        // order on the last two)
         const int wv = tid >> 6, nwv = NT >> 6, lane6 = tid & 63;
         if (wv == nwv - 1) {
             // exclusive scan of cnt[0..nc)
@@ -726,20 +727,13 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
                 carry += __builtin_amdgcn_readlane(x, 63);
             }
         } else if (wv == nwv - 2) {
-            // histogram of the record counts (bins 0..63)
+            // histogram of the record counts (bins 0..63), then the cells in
+            // descending record count: bin b starts after all larger bins;
+            // the cells a wave then sums together have similar loop lengths.
+            // One wave does both (its LDS operations complete in order), in
+            // the shadow of the crossers.
             for (int c = lane6; c < nc; c += 64) atomicAdd(hist + min(cnt[c], 63), 1);
-        }
-        __syncthreads();
-        PHASE_STAMP(2);
-        if (last && more && tid < fplane) put();
-        for (int r = tid; r < nr; r += NT) {
-            const int c = rc[r];
-            const int pos = atomicAdd(fill + c, 1);
-            idx[start[c] + pos] = (uint16_t)r;
-        }
-        if (wv == nwv - 2) {
-            // cells in descending record count: bin b starts after all larger
-            // bins; the cells a wave then sums together have similar loop lengths
+            __builtin_amdgcn_wave_barrier();
             const int h = hist[lane6];
             int x = h;
     #pragma unroll
@@ -750,6 +744,14 @@ __global__ void __launch_bounds__(CELLS_T) k_push_cells(TiledArgs a, CellsLds L)
             hist[lane6] = x - h;
             __builtin_amdgcn_wave_barrier();
             for (int c = lane6; c < nc; c += 64) perm[atomicAdd(hist + min(cnt[c], 63), 1)] = (uint16_t)c;
+        }
+        __syncthreads();
+        PHASE_STAMP(2);
+        if (last && more && tid < fplane) put();
+        for (int r = tid; r < nr; r += NT) {
+            const int c = rc[r];
+            const int pos = atomicAdd(fill + c, 1);
+            idx[start[c] + pos] = (uint16_t)r;
         }
         __syncthreads();
         PHASE_STAMP(3);
