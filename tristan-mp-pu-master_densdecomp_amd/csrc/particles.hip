@@ -470,33 +470,34 @@ __device__ __forceinline__ void cell_sums(int Q, int c, int r0, int n, int tx, i
         const int r = idx[rr];
         const float o = rec[axis * CELLS_R + r], d = rec[(3 + axis) * CELLS_R + r];
         const float q = rq[r];
-        float S6[6], dS6[6];
-        profile6<ORD>(o, d, S6, dS6);
+        // each lane forms A = S + dS/2 of its own axis, and the quad shares A
+        // and dS
+        float A6[6], dS6[6];
+        profile6<ORD>(o, d, A6, dS6);
+#pragma unroll
+        for (int u = 0; u < 6; ++u) A6[u] = A6[u] + 0.5f * dS6[u];
         float dSx[6], Ax[6], dSy[4], Ay[4];
 #pragma unroll
         for (int u = 0; u < 6; ++u) {
-            const float sx = quad_bcast<0>(S6[u]);
+            Ax[u] = quad_bcast<0>(A6[u]);
             dSx[u] = quad_bcast<0>(dS6[u]);
-            Ax[u] = sx + 0.5f * dSx[u];
         }
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const float sy = quad_bcast<1>(S6[1 + b]);
+            Ay[b] = quad_bcast<1>(A6[1 + b]);
             dSy[b] = quad_bcast<1>(dS6[1 + b]);
-            Ay[b] = sy + 0.5f * dSy[b];
         }
         float azc, dzc, cz0, cz1, cz2;
         {
-            float Sz[4], dSz[4];
+            float Az[4], dSz[4];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                Sz[b] = quad_bcast<2>(S6[1 + b]);
+                Az[b] = quad_bcast<2>(A6[1 + b]);
                 dSz[b] = quad_bcast<2>(dS6[1 + b]);
             }
             const float s0 = dSz[0], s1 = s0 + dSz[1], s2 = s1 + dSz[2];
-            const float S = Q == 0 ? Sz[0] : Q == 1 ? Sz[1] : Q == 2 ? Sz[2] : Sz[3];
             dzc = Q == 0 ? dSz[0] : Q == 1 ? dSz[1] : Q == 2 ? dSz[2] : dSz[3];
-            azc = S + 0.5f * dzc;
+            azc = Q == 0 ? Az[0] : Q == 1 ? Az[1] : Q == 2 ? Az[2] : Az[3];
             cz0 = q * s0;
             cz1 = q * s1;
             cz2 = q * s2;
