@@ -229,6 +229,14 @@ def test_push_deposit_vs_oracle_fp64(tmpg, dim, order):
         _push_deposit_case(tmpg, dim, order, (48, 40, 1), 8000)
 
 
+@pytest.mark.parametrize("order", [2, 3])
+def test_push_deposit_persistent_tiles(tmpg, order):
+    """64^3 cells: 1024 tiles of 8x8x4, more than one per resident
+    workgroup, so the persistent 3D push carries its fields, cleared
+    currents and record counters from tile to tile"""
+    _push_deposit_case(tmpg, 3, order, (64, 64, 64), 2 * 64 ** 3)
+
+
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_push_deposit_dense_tile(tmpg, order):
     """one 8x8x4 tile holding 5207 particles: more than the 3D push keeps
