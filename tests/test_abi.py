@@ -35,6 +35,17 @@ def test_python_binding_covers_header():
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libtmpg.so not built")
+def test_phase_cycles_needs_the_stamped_build():
+    """the product library has no phase stamps: tmpg_phase_cycles reports
+    TMPG_ESTATE (-5) without touching the device, and rejects null arrays"""
+    lib = ctypes.CDLL(LIB)
+    buf = (ctypes.c_ulonglong * 8)()
+    cnt = (ctypes.c_ulonglong * 8)()
+    assert lib.tmpg_phase_cycles(buf, cnt, 1) == -5
+    assert lib.tmpg_phase_cycles(None, cnt, 0) == -1
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libtmpg.so not built")
 def test_input_reader(tmp_path):
     lib = ctypes.CDLL(LIB)
     p = tmp_path / "input"
