@@ -237,6 +237,14 @@ def test_push_deposit_persistent_tiles(tmpg, order):
     _push_deposit_case(tmpg, 3, order, (64, 64, 64), 2 * 64 ** 3)
 
 
+def test_push_deposit_persistent_multipass(tmpg):
+    """64x64x32 cells at 6 particles per cell per species: 512 tiles
+    averaging more particles than the record space holds, so the persistent
+    order-3 push runs its multi-pass instantiation over several tiles per
+    workgroup"""
+    _push_deposit_case(tmpg, 3, 3, (64, 64, 32), 6 * 64 * 64 * 32)
+
+
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_push_deposit_dense_tile(tmpg, order):
     """one 8x8x4 tile holding 5207 particles: more than the 3D push keeps
